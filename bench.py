@@ -1,0 +1,150 @@
+"""Benchmark: implicit-Euler steps/s of the MI355X descent solver at rel-grad 1e-4.
+
+Default workload (one GPU): the 64x64x256 bar of 6,291,456 tets (Freudenthal split, edge 0.05 m,
+neo-Hookean E=1e5 Pa, nu=0.45, rho=1000, gravity -z, clamp x=0), dt = 1/30, iterating each step
+until |g_k| <= 1e-4 |g_0| (FP64 state and per-tet arithmetic).  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2a|c2b|c1|c0] [--mode relgrad|fixed]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+WORKLOADS = {
+    # name: (nx, ny, nz, material, twist_deg, jitter, fixed iterations)
+    "c0": (4, 4, 16, 0, 0.0, 0.0, 96),
+    "c1": (12, 12, 50, 0, 90.0, 0.01, 96),
+    "c2a": (64, 64, 256, 0, 0.0, 0.0, 64),
+    "c2b": (64, 64, 256, 1, 0.0, 0.0, 64),
+}
+
+
+def algorithmic_bytes(nt, nn):
+    """Compulsory HBM bytes of one descent iteration: 56 B per tet (4 int32 indices, 9 FP32
+    Dm^-1 entries, 1 FP32 volume) and 68 B per node (read x, x~, x_last, Hessian diagonal + m/h^2,
+    mass, mask; write x_next), FP32 / int32."""
+    return {"k_tet": 56.0 * nt, "k_node": 68.0 * nn}
+
+
+def cpu_baseline(rest, tets, mask, material, iters_per_step, grav, seconds=12.0):
+    """The oracle (FP64, OpenMP) timed on the host on a bounded sample: a fixed number of descent
+    iterations of the same mesh, scaled to steps/s with the GPU's iterations per step."""
+    from oracle import pyoracle as O
+
+    cores = min(16, len(os.sched_getaffinity(0)))
+    p = O.default_params(iterations=2, grav=grav, dt=1.0 / 30.0,
+                         profile=[(1, 0.96), (7, 0.991), (10, 0.9995)])
+    o = O.Solver(rest, tets, mask, params=p, precision=64, material=material, threads=cores)
+    t0 = time.perf_counter()
+    o.step()
+    t1 = time.perf_counter()
+    per_iter = (t1 - t0) / 2.0
+    n = max(2, min(200, int(seconds / max(per_iter, 1e-9))))
+    p.iterations = n
+    o.set_params(p)
+    o.set_state(x=rest.astype(np.float64))
+    t0 = time.perf_counter()
+    o.step()
+    dt = time.perf_counter() - t0
+    its = n / dt
+    return {"value": its / max(iters_per_step, 1.0), "unit": "steps/s", "cores": cores, "kind": "port",
+            "sample": f"{n} FP64 descent iterations of the same mesh in {dt:.1f} s "
+                      f"({its:.2f} it/s), divided by the GPU's {iters_per_step:.0f} iterations/step",
+            "iterations_per_s": its}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2a", choices=sorted(WORKLOADS))
+    ap.add_argument("--mode", default="relgrad", choices=["relgrad", "fixed"])
+    ap.add_argument("--tol", type=float, default=1e-4)
+    ap.add_argument("--max-iterations", type=int, default=40000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: the descent solver has no multi-GPU path in this build "
+              "(see DESIGN.md for how it would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import phybs_amd as P
+
+    nx, ny, nz, material, twist, jitter, fixed_iters = WORKLOADS[args.workload]
+    edge = 0.05
+    rest, tets = P.mesh_bar(nx, ny, nz, edge)
+    mask = (rest[:, 0] == 0).astype(np.int32)
+    x0 = P.mesh_twist(rest, nx, ny, nz, edge, twist, jitter, 1) if (twist or jitter) else rest
+    grav = (0.0, 0.0, -9.8)
+    relg = args.mode == "relgrad"
+    params = P.default_params(
+        dt=1.0 / 30.0, grav=grav, material=material, precision=P.FP64 if relg else P.FP32,
+        iterations=fixed_iters, relgrad_tol=args.tol if relg else 0.0, max_iterations=args.max_iterations,
+        profile=[(1, 0.96), (7, 0.991), (10, 0.9995)])
+    s = P.Solver(rest, tets, mask, params)
+    s.set_positions(x0)
+    for _ in range(args.warmup):
+        s.step()
+    s.set_profiling(True)
+    s.reset_kernel_times()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(s.step())
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    kt = s.kernel_times()
+    iters = sum(st.iterations for st in stats)
+    passes = sum(st.passes for st in stats)
+    nt, nn = len(tets), len(rest)
+    alg = algorithmic_bytes(nt, nn)
+    dom = max(("k_tet", "k_node"), key=lambda k: kt[k][0] * kt[k][1])
+    avg_ms = kt[dom][0]
+    achieved = alg[dom] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    steps_per_s = args.steps / elapsed
+    out = {
+        "metric": "steps/s at rel-grad 1e-4 (plain descent)" if relg else "steps/s (fixed iterations)",
+        "value": steps_per_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if relg else "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {nx}x{ny}x{nz} bar, {nt} tets, {nn} nodes, "
+                               f"{'stretch' if material else 'neo-Hookean'}, "
+                               + (f"rel-grad {args.tol:g}" if relg else f"{fixed_iters} iterations/step"),
+                   "dt": 1.0 / 30.0, "tets": nt, "nodes": nn},
+        "iterations_per_step": iters / args.steps,
+        "descent_iterations_per_s": iters / elapsed,
+        "rollbacks": sum(st.rollbacks for st in stats),
+        "converged_steps": sum(st.converged for st in stats),
+        "rel_grad_final": [st.g_norm / st.g0_norm if st.g0_norm > 0 else None for st in stats],
+        "kernel_avg_ms": {k: v[0] for k, v in kt.items()},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "algorithmic_bytes_per_launch": alg[dom]},
+        "learned_warm_start": None,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(rest, tets, mask, material, iters / args.steps, grav)
+    s.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

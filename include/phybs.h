@@ -1,0 +1,180 @@
+/*
+ * phybs.h -- C ABI of the MI355X (gfx950) elastic-body descent solver.
+ *
+ * This library replaces the GPU descent path of PhyBS (class CUDAInterface, FVM/CUDAInterface.h:20-102,
+ * FVM/CUDAInterface.cu:482-1007) and exposes the CPU twin's material calls
+ * (IsotropicNeohookeanMaterial / IsotropicMaterial) on the GPU.  All entry points take plain
+ * pointers and sizes; host buffers are caller-owned and copied in/out; device memory is owned
+ * by the solver handle.  Positions, velocities and forces cross the boundary as AoS float xyz
+ * triples (the layout of the reference's Eigen MatrixXf(3,N), TetMesh.h:90); tets as 4 int32
+ * per tet (MatrixXi(4,T), TetMesh.h:93).
+ *
+ * Errors: every function returns PBS_OK (0) or a negative PBS_ERR_* code; pbs_last_error()
+ * returns the message of the calling thread's last failure.  A handle is not re-entrant: use
+ * one handle per host thread.  Work runs on the handle's own HIP stream.
+ */
+#ifndef PHYBS_H
+#define PHYBS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBS_ABI_VERSION 1
+
+#define PBS_OK 0
+#define PBS_ERR_ARG -1   /* invalid argument (null pointer, bad size, bad enum) */
+#define PBS_ERR_HIP -2   /* HIP runtime error (no device, allocation failure, launch failure) */
+#define PBS_ERR_MESH -3  /* invalid mesh (index out of range, inverted or degenerate rest tet) */
+#define PBS_ERR_IO -4    /* file could not be read or written */
+#define PBS_ERR_STATE -5 /* call not valid in the handle's current state */
+
+/* Semantics of the descent loop.  The reference ships two versions that differ. */
+#define PBS_SEM_REF_GPU 0 /* CUDAInterface::doDescentOpt (CUDAInterface.cu:911-1007) -- default */
+#define PBS_SEM_REF_CPU 1 /* DescentOptimize::doDescentOpt (DescentOptimize.cpp:91-188): gravity and
+                             external forces counted twice in the gradient, energy clamp E>1e-6 */
+
+#define PBS_FP32 32 /* FP32 state and per-tet arithmetic (the reference's precision) */
+#define PBS_FP64 64 /* FP64 state and per-tet arithmetic, FP32 mesh data (needed for rel-grad <= 1e-4) */
+
+#define PBS_MAT_NEOHOOKEAN 0 /* IsotropicNeohookeanMaterial (IsotropicNeohookeanMaterial.cpp) */
+#define PBS_MAT_STRETCH 1    /* psi = sum_i mu (s_i - 1)^2 + lambda/2 (J - 1)^2 (no reference class) */
+
+typedef struct pbs_solver pbs_solver;
+
+/* Every constant the reference hard-codes, with its value as the default. */
+typedef struct pbs_params {
+  float dt;             /* time step, 0.03 (CUDAInterface.cu:644; DescentOptimize.cpp:5) */
+  float grav[3];        /* gravity acceleration, (0,-9.8,0) (TetMesh.cpp:205) */
+  float youngs;         /* E, 5e5 in TetMesh.cpp:185 -- default here 1e5 */
+  float nu;             /* Poisson ratio 0.45 */
+  float density;        /* 1000 */
+  float fixed_mass;     /* mass given to constrained nodes, 1e8 (TetMesh.cpp:215) */
+  float alpha0;         /* initial step length 0.1 (CUDAInterface.cu:914) */
+  float alpha_decay;    /* 0.7 on each rollback (:948) */
+  float alpha_min;      /* stop when alpha < 0.01 (:919) */
+  int hd_refresh;       /* Hessian-diagonal refresh period 32 (:921) */
+  int energy_check;     /* energy check period 8 (:942) */
+  float damping;        /* velocity damping 0.99 (:637) */
+  float init_momentum;  /* initial-guess momentum 0.2 (:491) */
+  int profile_n;        /* Chebyshev profile length 3 (:732-738) */
+  int profile_k[8];     /* 1, 7, 10 */
+  float profile_rho[8]; /* 0.96, 0.991, 0.999 */
+  int semantics;        /* PBS_SEM_* */
+  int precision;        /* PBS_FP32 / PBS_FP64 */
+  int material;         /* PBS_MAT_* */
+  int iterations;       /* descent iterations per step, 96 (fvm.cpp:187) */
+  double relgrad_tol;   /* 0: fixed iterations; > 0: iterate until |g_k| <= tol |g_0| (free DOFs, FP64) */
+  int max_iterations;   /* iteration cap in rel-grad mode */
+  int block_size;       /* threads per workgroup, 0 = tuned default */
+} pbs_params;
+
+typedef struct pbs_step_stats {
+  int passes;       /* loop bodies run (accepted iterations + rollbacks) */
+  int iterations;   /* accepted iterations (the reference's final k) */
+  int rollbacks;    /* line-search rollbacks (CUDAInterface.cu:946-958) */
+  int converged;    /* rel-grad mode: tolerance met */
+  int used_learned; /* learned warm start accepted (not rejected by the gradient test) */
+  float alpha;      /* final step length */
+  double energy;    /* last accepted incremental-potential value (FP64 sum) */
+  double g0_norm;   /* |g| at the plain initial guess, free DOFs */
+  double g_norm;    /* |g| at the last evaluated iterate */
+  double gpu_ms;    /* device time of the step (HIP events on the handle's stream) */
+} pbs_step_stats;
+
+/* ---- parameters / errors ---- */
+int pbs_default_params(pbs_params *p);
+const char *pbs_last_error(void);
+int pbs_abi_version(void);
+
+/* ---- solver lifetime ----
+ * Replaces CUDAInterface::CUDAInterface(int num_nodes, const float *nodes, const float *restPoses,
+ * const int *constraintsMask, int num_tets, const int *tets, float youngs, float nu, float density,
+ * int csr_nnz, ...) (CUDAInterface.h:23-25, CUDAInterface.cu:641-740).  The CSR arguments of the
+ * reference serve its backward-Euler path only and are not taken.  As in the reference, the
+ * initial positions are the rest positions (reset() copies restPoses, CUDAInterface.cu:776-782);
+ * `nodes` is accepted for signature parity and applied after the reset when non-NULL.
+ * Rest tets must have det(Dm) > 0 (the AN convention of TetMesh.cpp:279-294). */
+int pbs_create(int num_nodes, const float *nodes, const float *rest, const int *constraint_mask, int num_tets,
+               const int *tets, const pbs_params *p, pbs_solver **out);
+/* ~CUDAInterface (CUDAInterface.cu:742-769). */
+void pbs_destroy(pbs_solver *s);
+/* Change solver parameters (material constants and gravity re-run the setup kernels). */
+int pbs_set_params(pbs_solver *s, const pbs_params *p);
+int pbs_get_params(const pbs_solver *s, pbs_params *p);
+int pbs_sizes(const pbs_solver *s, int *num_nodes, int *num_tets);
+
+/* Rest-shape data computed by the GPU setup kernels (compute_DmInv_ANs_Mass and the ctor,
+ * CUDAInterface.cu:273-325, :704-725): dminv 9T (row-major Dm^-1 per tet), vol T, mass N
+ * (constrained nodes hold fixed_mass), gforce 3N AoS (mass before the override * grav).
+ * Any pointer may be NULL. */
+int pbs_get_rest_data(pbs_solver *s, float *dminv, float *vol, float *mass, float *gforce);
+
+/* ---- state ---- */
+/* CUDAInterface::reset (CUDAInterface.cu:776-782): positions := rest, velocities := 0. */
+int pbs_reset(pbs_solver *s);
+/* CUDAInterface::updateNodePositions (CUDAInterface.cu:771-774). */
+int pbs_set_positions(pbs_solver *s, const float *xyz);
+int pbs_get_positions(pbs_solver *s, float *xyz);
+int pbs_get_positions_f64(pbs_solver *s, double *xyz);
+int pbs_set_velocities(pbs_solver *s, const float *v, const float *v_last);
+int pbs_get_velocities(pbs_solver *s, float *v, float *v_last);
+
+/* ---- the hot path ----
+ * One implicit-Euler step by Jacobi-preconditioned descent with Chebyshev acceleration and the
+ * energy rollback line search.  ext: 3N AoS external forces or NULL; out: 3N AoS positions
+ * after the step or NULL; st: optional statistics. */
+int pbs_step(pbs_solver *s, const float *ext, float *out, pbs_step_stats *st);
+/* CUDAInterface::doDescentOpt(float h, int iterations, float *hostExternForces, float *hostNode)
+ * (CUDAInterface.h:35, CUDAInterface.cu:911-1007).  As in the reference, h is ignored (the
+ * handle's dt is used) and ext must not be NULL. */
+int pbs_do_descent_opt(pbs_solver *s, float h, int iterations, const float *ext, float *out);
+
+/* ---- material calls on the GPU ----
+ * IsotropicNeohookeanMaterial::computeInnerForceFromPos / computeElasticEnergyFromPos
+ * (IsotropicNeohookeanMaterial.cpp:93-131) at positions pos (3N AoS).  Outputs (any may be NULL):
+ * tet_forces 12T (forces on the tet's 4 nodes, xyz each), node_forces 3N (elastic only: the
+ * reference adds gravity, external and fixed-node spring forces there, TetMesh.cpp:580-586),
+ * tet_energy T (psi * vol with the semantics' clamp). */
+int pbs_compute_forces(pbs_solver *s, const float *pos, float *tet_forces, float *node_forces, float *tet_energy);
+/* Diagonal of IsotropicMaterial::computeGlobalStiffnessMatrixFromPos (IsotropicMaterial.cpp:646-682),
+ * i.e. of K = -df/dx: node_hd 3N, tet_hd 12T (either may be NULL). */
+int pbs_compute_hessian_diag(pbs_solver *s, const float *pos, float *node_hd, float *tet_hd);
+
+/* ---- measurement ---- */
+/* Average device time per launch (ms) of each kernel of the last pbs_step calls, measured with
+ * HIP events on the handle's stream: names[i] / avg_ms[i] / launches[i] for i < *count. */
+int pbs_kernel_times(pbs_solver *s, int max, const char **names, double *avg_ms, long *launches, int *count);
+int pbs_reset_kernel_times(pbs_solver *s);
+int pbs_set_profiling(pbs_solver *s, int on);
+
+/* ---- host mesh utilities (no GPU work) ---- */
+/* Bar of nx*ny*nz cubes of size edge, 6 tets per cube (Freudenthal split, det(Dm) > 0).
+ * nodes: 3*(nx+1)(ny+1)(nz+1) floats, tets: 24*nx*ny*nz ints; node id = i + (nx+1)(j + (ny+1)k). */
+int pbs_mesh_bar(int nx, int ny, int nz, float edge, float *nodes, int *tets);
+/* Twist about the bar's z axis (theta = twist_deg * z / (nz*edge)) plus uniform +-jitter*edge
+ * SplitMix64(seed) noise per coordinate, computed in FP64 and rounded (in place, 3N AoS). */
+int pbs_mesh_twist(int nx, int ny, int nz, float edge, double twist_deg, double jitter, uint64_t seed,
+                   int num_nodes, float *nodes);
+/* TetGen .node/.ele/.face (TetMesh.cpp:14-151).  Counts first, then load; positions are
+ * multiplied by scale (the reference uses 0.1, TetMesh.cpp:38).  faces may be NULL. */
+int pbs_tetgen_count(const char *prefix, int *num_nodes, int *num_tets, int *num_faces);
+int pbs_tetgen_load(const char *prefix, float scale, float *nodes, int *tets, int *faces);
+/* .node export / import (TetMesh.cpp:558-570 / :70-94, no scaling on import). */
+int pbs_node_write(const char *path, int num_nodes, const float *nodes);
+int pbs_node_read(const char *path, int num_nodes, float *nodes);
+/* Stiffness pattern of IsotropicMaterial::allocateGlobalStiffnessMatrix (IsotropicMaterial.cpp:554-644):
+ * returns nnz (>= 0) or an error; when rowptr/colidx are non-NULL fills rowptr (3N+1), colidx
+ * (nnz), diag_idx (3N, getDiagonalIdx) and kid (48T, getKIDinCSRval). */
+int64_t pbs_csr_pattern(int num_nodes, int num_tets, const int *tets, int *rowptr, int *colidx, int *diag_idx,
+                        int *kid);
+/* Tet incidence per node in ascending tet order (ProjDynamic::initLocalProjection,
+ * ProjDynamic.cpp:25-45): offsets N+1, entries 4T = 12*tet + 3*local. */
+int pbs_incidence(int num_nodes, int num_tets, const int *tets, int *offsets, int *entries);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,293 @@
+"""phybs_amd -- Python binding of the MI355X descent solver's C ABI (include/phybs.h).
+
+The solver itself is libphybs.so (HIP kernels for gfx950 + the C ABI).  This module only
+marshals numpy arrays through ctypes; there is no Python or CPU fallback: if the native
+library is missing, importing the binding raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libphybs.so")
+
+OK = 0
+ERR_ARG, ERR_HIP, ERR_MESH, ERR_IO, ERR_STATE = -1, -2, -3, -4, -5
+SEM_REF_GPU, SEM_REF_CPU = 0, 1
+FP32, FP64 = 32, 64
+MAT_NEOHOOKEAN, MAT_STRETCH = 0, 1
+
+
+class PhybsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"phybs error {code}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("dt", C.c_float), ("grav", C.c_float * 3), ("youngs", C.c_float), ("nu", C.c_float),
+        ("density", C.c_float), ("fixed_mass", C.c_float), ("alpha0", C.c_float),
+        ("alpha_decay", C.c_float), ("alpha_min", C.c_float), ("hd_refresh", C.c_int),
+        ("energy_check", C.c_int), ("damping", C.c_float), ("init_momentum", C.c_float),
+        ("profile_n", C.c_int), ("profile_k", C.c_int * 8), ("profile_rho", C.c_float * 8),
+        ("semantics", C.c_int), ("precision", C.c_int), ("material", C.c_int),
+        ("iterations", C.c_int), ("relgrad_tol", C.c_double), ("max_iterations", C.c_int),
+        ("block_size", C.c_int),
+    ]
+
+
+class StepStats(C.Structure):
+    _fields_ = [
+        ("passes", C.c_int), ("iterations", C.c_int), ("rollbacks", C.c_int), ("converged", C.c_int),
+        ("used_learned", C.c_int), ("alpha", C.c_float), ("energy", C.c_double), ("g0_norm", C.c_double),
+        ("g_norm", C.c_double), ("gpu_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# name -> (restype, argtypes); every symbol include/phybs.h declares.
+_P = C.c_void_p
+SIGNATURES = {
+    "pbs_default_params": (C.c_int, [_P]),
+    "pbs_last_error": (C.c_char_p, []),
+    "pbs_abi_version": (C.c_int, []),
+    "pbs_create": (C.c_int, [C.c_int, _P, _P, _P, C.c_int, _P, _P, C.POINTER(_P)]),
+    "pbs_destroy": (None, [_P]),
+    "pbs_set_params": (C.c_int, [_P, _P]),
+    "pbs_get_params": (C.c_int, [_P, _P]),
+    "pbs_sizes": (C.c_int, [_P, _P, _P]),
+    "pbs_get_rest_data": (C.c_int, [_P, _P, _P, _P, _P]),
+    "pbs_reset": (C.c_int, [_P]),
+    "pbs_set_positions": (C.c_int, [_P, _P]),
+    "pbs_get_positions": (C.c_int, [_P, _P]),
+    "pbs_get_positions_f64": (C.c_int, [_P, _P]),
+    "pbs_set_velocities": (C.c_int, [_P, _P, _P]),
+    "pbs_get_velocities": (C.c_int, [_P, _P, _P]),
+    "pbs_step": (C.c_int, [_P, _P, _P, _P]),
+    "pbs_do_descent_opt": (C.c_int, [_P, C.c_float, C.c_int, _P, _P]),
+    "pbs_compute_forces": (C.c_int, [_P, _P, _P, _P, _P]),
+    "pbs_compute_hessian_diag": (C.c_int, [_P, _P, _P, _P]),
+    "pbs_kernel_times": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),
+    "pbs_reset_kernel_times": (C.c_int, [_P]),
+    "pbs_set_profiling": (C.c_int, [_P, C.c_int]),
+    "pbs_mesh_bar": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_float, _P, _P]),
+    "pbs_mesh_twist": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_float, C.c_double, C.c_double, C.c_uint64,
+                                 C.c_int, _P]),
+    "pbs_tetgen_count": (C.c_int, [C.c_char_p, _P, _P, _P]),
+    "pbs_tetgen_load": (C.c_int, [C.c_char_p, C.c_float, _P, _P, _P]),
+    "pbs_node_write": (C.c_int, [C.c_char_p, C.c_int, _P]),
+    "pbs_node_read": (C.c_int, [C.c_char_p, C.c_int, _P]),
+    "pbs_csr_pattern": (C.c_int64, [C.c_int, C.c_int, _P, _P, _P, _P, _P]),
+    "pbs_incidence": (C.c_int, [C.c_int, C.c_int, _P, _P, _P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libphybs.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PhybsError(ERR_STATE, f"{LIB_PATH} not built: run __graft_entry__.build() or make -C phybs_amd/csrc")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc):
+    if rc < 0:
+        raise PhybsError(rc, lib().pbs_last_error().decode())
+    return rc
+
+
+def default_params(**kw):
+    p = Params()
+    _check(lib().pbs_default_params(C.byref(p)))
+    for k, v in kw.items():
+        if k == "grav":
+            for i in range(3):
+                p.grav[i] = v[i]
+        elif k == "profile":
+            p.profile_n = len(v)
+            for i, (kk, rr) in enumerate(v):
+                p.profile_k[i] = kk
+                p.profile_rho[i] = rr
+        else:
+            if not hasattr(p, k):
+                raise AttributeError(k)
+            setattr(p, k, v)
+    return p
+
+
+def _f32(a, n=None):
+    a = np.ascontiguousarray(a, np.float32)
+    return a.reshape(-1) if n is None else a.reshape(n)
+
+
+# ---- host mesh utilities ----
+def mesh_bar(nx, ny, nz, edge=0.05):
+    nn = (nx + 1) * (ny + 1) * (nz + 1)
+    nodes = np.zeros((nn, 3), np.float32)
+    tets = np.zeros((6 * nx * ny * nz, 4), np.int32)
+    _check(lib().pbs_mesh_bar(nx, ny, nz, edge, _p(nodes), _p(tets)))
+    return nodes, tets
+
+
+def mesh_twist(nodes, nx, ny, nz, edge=0.05, twist_deg=90.0, jitter=0.01, seed=1):
+    x = np.array(nodes, np.float32, copy=True, order="C")
+    _check(lib().pbs_mesh_twist(nx, ny, nz, edge, twist_deg, jitter, seed, len(x), _p(x)))
+    return x
+
+
+def tetgen_load(prefix, scale=1.0, faces=False):
+    nn, nt, nf = C.c_int(), C.c_int(), C.c_int()
+    _check(lib().pbs_tetgen_count(prefix.encode(), C.byref(nn), C.byref(nt), C.byref(nf)))
+    nodes = np.zeros((nn.value, 3), np.float32)
+    tets = np.zeros((nt.value, 4), np.int32)
+    fc = np.zeros((nf.value, 3), np.int32) if faces else None
+    _check(lib().pbs_tetgen_load(prefix.encode(), scale, _p(nodes), _p(tets), _p(fc)))
+    return (nodes, tets, fc) if faces else (nodes, tets)
+
+
+def node_write(path, nodes):
+    nodes = _f32(nodes)
+    _check(lib().pbs_node_write(path.encode(), len(nodes) // 3, _p(nodes)))
+
+
+def node_read(path, nn):
+    out = np.zeros((nn, 3), np.float32)
+    _check(lib().pbs_node_read(path.encode(), nn, _p(out)))
+    return out
+
+
+def csr_pattern(nn, tets):
+    tets = np.ascontiguousarray(tets, np.int32).reshape(-1, 4)
+    nnz = _check(lib().pbs_csr_pattern(nn, len(tets), _p(tets), None, None, None, None))
+    rowptr = np.zeros(3 * nn + 1, np.int32)
+    col = np.zeros(nnz, np.int32)
+    diag = np.zeros(3 * nn, np.int32)
+    kid = np.zeros(48 * len(tets), np.int32)
+    _check(lib().pbs_csr_pattern(nn, len(tets), _p(tets), _p(rowptr), _p(col), _p(diag), _p(kid)))
+    return rowptr, col, diag, kid
+
+
+def incidence(nn, tets):
+    tets = np.ascontiguousarray(tets, np.int32).reshape(-1, 4)
+    off = np.zeros(nn + 1, np.int32)
+    ent = np.zeros(4 * len(tets), np.int32)
+    _check(lib().pbs_incidence(nn, len(tets), _p(tets), _p(off), _p(ent)))
+    return off, ent
+
+
+class Solver:
+    """GPU descent solver handle (pbs_create ... pbs_destroy)."""
+
+    def __init__(self, rest, tets, mask=None, params=None, nodes=None):
+        self.rest = _f32(rest, (-1, 3))
+        self.tets = np.ascontiguousarray(tets, np.int32).reshape(-1, 4)
+        self.nn, self.nt = len(self.rest), len(self.tets)
+        self.mask = None if mask is None else np.ascontiguousarray(mask, np.int32)
+        self.params = params if params is not None else default_params()
+        h = C.c_void_p()
+        x0 = None if nodes is None else _f32(nodes, (-1, 3))
+        _check(lib().pbs_create(self.nn, _p(x0), _p(self.rest), _p(self.mask), self.nt, _p(self.tets),
+                                C.byref(self.params), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pbs_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def set_params(self, params):
+        _check(lib().pbs_set_params(self.h, C.byref(params)))
+        self.params = params
+
+    def reset(self):
+        _check(lib().pbs_reset(self.h))
+
+    def set_positions(self, x):
+        x = _f32(x)
+        _check(lib().pbs_set_positions(self.h, _p(x)))
+
+    def positions(self, f64=False):
+        out = np.zeros((self.nn, 3), np.float64 if f64 else np.float32)
+        _check((lib().pbs_get_positions_f64 if f64 else lib().pbs_get_positions)(self.h, _p(out)))
+        return out
+
+    def set_velocities(self, v=None, v_last=None):
+        v = None if v is None else _f32(v)
+        vl = None if v_last is None else _f32(v_last)
+        _check(lib().pbs_set_velocities(self.h, _p(v), _p(vl)))
+
+    def velocities(self):
+        v = np.zeros((self.nn, 3), np.float32)
+        vl = np.zeros((self.nn, 3), np.float32)
+        _check(lib().pbs_get_velocities(self.h, _p(v), _p(vl)))
+        return v, vl
+
+    def step(self, ext=None, want_positions=False):
+        st = StepStats()
+        e = None if ext is None else _f32(ext)
+        out = np.zeros((self.nn, 3), np.float32) if want_positions else None
+        _check(lib().pbs_step(self.h, _p(e), _p(out), C.byref(st)))
+        return (st, out) if want_positions else st
+
+    def do_descent_opt(self, h, iterations, ext):
+        e = _f32(ext)
+        out = np.zeros((self.nn, 3), np.float32)
+        _check(lib().pbs_do_descent_opt(self.h, h, iterations, _p(e), _p(out)))
+        return out
+
+    def compute_forces(self, pos):
+        pos = _f32(pos)
+        tf = np.zeros((self.nt, 12), np.float32)
+        nf = np.zeros((self.nn, 3), np.float32)
+        te = np.zeros(self.nt, np.float32)
+        _check(lib().pbs_compute_forces(self.h, _p(pos), _p(tf), _p(nf), _p(te)))
+        return tf, nf, te
+
+    def compute_hessian_diag(self, pos):
+        pos = _f32(pos)
+        nh = np.zeros((self.nn, 3), np.float32)
+        th = np.zeros((self.nt, 12), np.float32)
+        _check(lib().pbs_compute_hessian_diag(self.h, _p(pos), _p(nh), _p(th)))
+        return nh, th
+
+    def rest_data(self):
+        dminv = np.zeros((self.nt, 9), np.float32)
+        vol = np.zeros(self.nt, np.float32)
+        mass = np.zeros(self.nn, np.float32)
+        gf = np.zeros((self.nn, 3), np.float32)
+        _check(lib().pbs_get_rest_data(self.h, _p(dminv), _p(vol), _p(mass), _p(gf)))
+        return dict(dminv=dminv, vol=vol, mass=mass, gforce=gf)
+
+    def set_profiling(self, on=True):
+        _check(lib().pbs_set_profiling(self.h, 1 if on else 0))
+
+    def reset_kernel_times(self):
+        _check(lib().pbs_reset_kernel_times(self.h))
+
+    def kernel_times(self):
+        n = 16
+        names = (C.c_char_p * n)()
+        avg = np.zeros(n)
+        cnt = np.zeros(n, np.int64)
+        c = C.c_int()
+        _check(lib().pbs_kernel_times(self.h, n, C.cast(names, C.c_void_p), _p(avg), _p(cnt), C.byref(c)))
+        return {names[i].decode(): (float(avg[i]), int(cnt[i])) for i in range(c.value)}

@@ -1,0 +1,641 @@
+// pbs_capi.cpp -- C ABI of the descent solver (include/phybs.h): handle lifetime, state I/O,
+// the step driver and the GPU material calls.  Replaces CUDAInterface's host side
+// (CUDAInterface.cu:641-1007).  Everything runs on the handle's stream; the only host round
+// trip inside a step is one read of the 128-byte device control block after the planned passes
+// (the reference reads an energy every 8 iterations, CUDAInterface.cu:942-944).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pbs_host.h"
+#include "pbs_internal.h"
+#include "phybs.h"
+
+namespace pbs {
+static thread_local std::string g_err;
+int set_error(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+}  // namespace pbs
+
+using namespace pbs;
+
+#define HIPCHK(expr)                                                                                  \
+  do {                                                                                                \
+    hipError_t e_ = (expr);                                                                           \
+    if (e_ != hipSuccess) return set_error(PBS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+namespace {
+enum KernelId { K_TET = 0, K_NODE, K_CONTROL, K_INIT, K_FINAL, K_COUNT };
+const char *kKernelNames[K_COUNT] = {"k_tet", "k_node", "k_control", "k_step_init", "k_step_final"};
+}  // namespace
+
+struct pbs_solver {
+  int nn = 0, nt = 0;
+  pbs_params p{};
+  DevParams dp{};
+  int bs = 256, ntb = 0, nnb = 0;
+  hipStream_t st = nullptr;
+  int device = 0;
+  DevMesh<float> m32{};
+  DevMesh<double> m64{};
+  std::vector<float> rest_h;
+  float *stage = nullptr;     // 3N float AoS staging
+  double *stage64 = nullptr;  // 3N double AoS staging
+  std::vector<void *> allocs;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool profiling = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<int, int>> pending;  // (kernel id, first event index)
+  int pool_used = 0;
+  double kt_ms[K_COUNT] = {0};
+  long kt_n[K_COUNT] = {0};
+};
+
+template <typename R> static DevMesh<R> &mesh(pbs_solver *s);
+template <> DevMesh<float> &mesh<float>(pbs_solver *s) { return s->m32; }
+template <> DevMesh<double> &mesh<double>(pbs_solver *s) { return s->m64; }
+
+template <typename T> static int dalloc(pbs_solver *s, T **p, size_t n) {
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return set_error(PBS_ERR_HIP, std::string("hipMalloc(") + std::to_string(n * sizeof(T)) + " B): " + hipGetErrorString(e));
+  }
+  s->allocs.push_back(q);
+  *p = (T *)q;
+  return PBS_OK;
+}
+
+static int validate_params(const pbs_params *p) {
+  if (!p) return set_error(PBS_ERR_ARG, "null params");
+  if (p->precision != PBS_FP32 && p->precision != PBS_FP64) return set_error(PBS_ERR_ARG, "precision must be PBS_FP32 or PBS_FP64");
+  if (p->semantics != PBS_SEM_REF_GPU && p->semantics != PBS_SEM_REF_CPU) return set_error(PBS_ERR_ARG, "bad semantics");
+  if (p->material != PBS_MAT_NEOHOOKEAN && p->material != PBS_MAT_STRETCH) return set_error(PBS_ERR_ARG, "bad material");
+  if (!(p->dt > 0) || p->hd_refresh <= 0 || p->energy_check <= 0 || p->iterations < 0 || p->profile_n < 0 || p->profile_n > 8)
+    return set_error(PBS_ERR_ARG, "bad loop parameters");
+  if (!(p->youngs > 0) || !(p->nu > -1.0f && p->nu < 0.5f) || !(p->density > 0)) return set_error(PBS_ERR_ARG, "bad material constants");
+  if (p->relgrad_tol < 0 || (p->relgrad_tol > 0 && p->max_iterations <= 0)) return set_error(PBS_ERR_ARG, "bad rel-grad settings");
+  if (p->block_size != 0 && p->block_size != 64 && p->block_size != 128 && p->block_size != 256)
+    return set_error(PBS_ERR_ARG, "block_size must be 0, 64, 128 or 256");
+  return PBS_OK;
+}
+
+static void make_devparams(pbs_solver *s) {
+  const pbs_params &p = s->p;
+  DevParams &d = s->dp;
+  d.nn = s->nn;
+  d.nt = s->nt;
+  d.h = p.dt;
+  // mu = 0.5 E / (1 + nu), lambda = 2 mu / (1 - 2 nu) (CUDAInterface.cu:704-705)
+  d.mu = (float)(0.5 * (double)p.youngs / (double)(1 + p.nu));
+  d.lam = d.mu * 2 / (1 - 2 * p.nu);
+  d.density = p.density;
+  d.fixed_mass = p.fixed_mass;
+  for (int i = 0; i < 3; ++i) d.grav[i] = p.grav[i];
+  d.alpha0 = p.alpha0;
+  d.alpha_decay = p.alpha_decay;
+  d.alpha_min = p.alpha_min;
+  d.hd_refresh = p.hd_refresh;
+  d.energy_check = p.energy_check;
+  d.damping = p.damping;
+  d.init_momentum = p.init_momentum;
+  d.profile_n = p.profile_n;
+  for (int i = 0; i < 8; ++i) {
+    d.profile_k[i] = p.profile_k[i];
+    d.profile_rho[i] = p.profile_rho[i];
+  }
+  d.semantics = p.semantics;
+  d.material = p.material;
+  d.clamp = p.semantics == PBS_SEM_REF_CPU ? 2 : 1;
+  d.ext_scale = p.semantics == PBS_SEM_REF_CPU ? 2.0f : 1.0f;
+  d.has_ext = 0;
+  d.iterations = p.relgrad_tol > 0 ? p.max_iterations : p.iterations;
+  d.relgrad_tol = p.relgrad_tol;
+  s->bs = p.block_size ? p.block_size : 256;
+  s->ntb = (s->nt + s->bs - 1) / s->bs;
+  s->nnb = (s->nn + s->bs - 1) / s->bs;
+}
+
+// ---- kernel timing ----
+static int timed_begin(pbs_solver *s, int kid) {
+  if (!s->profiling) return -1;
+  if (s->pool_used + 2 > (int)s->pool.size()) {
+    for (int i = 0; i < 64; ++i) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return -1;
+      s->pool.push_back(e);
+    }
+  }
+  int i = s->pool_used;
+  s->pool_used += 2;
+  (void)hipEventRecord(s->pool[i], s->st);
+  s->pending.push_back({kid, i});
+  return i;
+}
+static void timed_end(pbs_solver *s, int i) {
+  if (i >= 0) (void)hipEventRecord(s->pool[i + 1], s->st);
+}
+static void timed_collect(pbs_solver *s) {
+  for (auto &pr : s->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, s->pool[pr.second], s->pool[pr.second + 1]) == hipSuccess) {
+      s->kt_ms[pr.first] += ms;
+      s->kt_n[pr.first] += 1;
+    }
+  }
+  s->pending.clear();
+  s->pool_used = 0;
+}
+
+// ---- state helpers ----
+template <typename R> static int upload_planes(pbs_solver *s, const float *aos, R *planes) {
+  HIPCHK(hipMemcpyAsync(s->stage, aos, sizeof(float) * 3 * (size_t)s->nn, hipMemcpyHostToDevice, s->st));
+  Launch<R>::aos_to_planes(s->stage, planes, s->nn, s->st);
+  HIPCHK(hipGetLastError());
+  return PBS_OK;
+}
+template <typename R> static int download_planes(pbs_solver *s, const R *planes, float *aos) {
+  Launch<R>::planes_to_aos(planes, s->stage, s->nn, s->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(aos, s->stage, sizeof(float) * 3 * (size_t)s->nn, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return PBS_OK;
+}
+
+template <typename R> static int reset_impl(pbs_solver *s) {
+  DevMesh<R> &m = mesh<R>(s);
+  int rc = upload_planes<R>(s, s->rest_h.data(), m.xpos);
+  if (rc) return rc;
+  HIPCHK(hipMemsetAsync(m.v, 0, sizeof(R) * 3 * (size_t)s->nn, s->st));
+  HIPCHK(hipMemsetAsync(m.vl, 0, sizeof(R) * 3 * (size_t)s->nn, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return PBS_OK;
+}
+
+template <typename R> static int alloc_impl(pbs_solver *s) {
+  DevMesh<R> &m = mesh<R>(s);
+  const size_t N = s->nn, T = std::max(s->nt, 1);
+  int rc = 0;
+  rc |= dalloc(s, &m.tets, T);
+  rc |= dalloc(s, &m.rest, 3 * N);
+  rc |= dalloc(s, &m.dmi, 9 * T);
+  rc |= dalloc(s, &m.vol, T);
+  rc |= dalloc(s, &m.inc_off, N + 1);
+  rc |= dalloc(s, &m.inc, 4 * T);
+  rc |= dalloc(s, &m.mask, N);
+  rc |= dalloc(s, &m.mass, N);
+  rc |= dalloc(s, &m.gforce, 3 * N);
+  rc |= dalloc(s, &m.ext, 3 * N);
+  rc |= dalloc(s, &m.hd, 3 * N);
+  rc |= dalloc(s, &m.frec, 12 * T);
+  rc |= dalloc(s, &m.hrec, 12 * T);
+  rc |= dalloc(s, &m.xt, 3 * N);
+  rc |= dalloc(s, &m.xpos, 3 * N);
+  rc |= dalloc(s, &m.v, 3 * N);
+  rc |= dalloc(s, &m.vl, 3 * N);
+  for (int i = 0; i < 4; ++i) rc |= dalloc(s, &m.ring.b[i], 3 * N);
+  // partial-sum slots for the smallest supported block (64 threads)
+  rc |= dalloc(s, &m.etet, T / 64 + 2);
+  rc |= dalloc(s, &m.enode, N / 64 + 2);
+  rc |= dalloc(s, &m.gpart, N / 64 + 2);
+  rc |= dalloc(s, &m.ctrl, 1);
+  rc |= dalloc(s, &m.flag, 1);
+  rc |= dalloc(s, &m.tete, T);
+  rc |= dalloc(s, &s->stage, 3 * N);
+  rc |= dalloc(s, &s->stage64, 3 * N);
+  return rc ? PBS_ERR_HIP : PBS_OK;
+}
+
+template <typename R> static int setup_impl(pbs_solver *s, const int *tets, const int *mask, const std::vector<int> &off,
+                                            const std::vector<int> &slots) {
+  DevMesh<R> &m = mesh<R>(s);
+  const size_t N = s->nn;
+  std::vector<float> planes(3 * N);
+  for (size_t i = 0; i < N; ++i)
+    for (int d = 0; d < 3; ++d) planes[d * N + i] = s->rest_h[3 * i + d];
+  if (s->nt > 0) HIPCHK(hipMemcpyAsync(m.tets, tets, sizeof(int) * 4 * (size_t)s->nt, hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemcpyAsync(m.rest, planes.data(), sizeof(float) * 3 * N, hipMemcpyHostToDevice, s->st));
+  std::vector<int> mk(N, 0);
+  if (mask)
+    for (size_t i = 0; i < N; ++i) mk[i] = mask[i] ? 1 : 0;
+  HIPCHK(hipMemcpyAsync(m.mask, mk.data(), sizeof(int) * N, hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemcpyAsync(m.inc_off, off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, s->st));
+  if (s->nt > 0) HIPCHK(hipMemcpyAsync(m.inc, slots.data(), sizeof(int) * 4 * (size_t)s->nt, hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemsetAsync(m.flag, 0, sizeof(int), s->st));
+  HIPCHK(hipMemsetAsync(m.hd, 0, sizeof(float) * 3 * N, s->st));
+  HIPCHK(hipMemsetAsync(m.ext, 0, sizeof(float) * 3 * N, s->st));
+  Launch<R>::setup_tets(m, s->dp, 256, s->st);
+  HIPCHK(hipGetLastError());
+  Launch<R>::setup_nodes(m, s->dp, 256, s->st);
+  HIPCHK(hipGetLastError());
+  int flag = 0;
+  HIPCHK(hipMemcpyAsync(&flag, m.flag, sizeof(int), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  if (flag & 1) return set_error(PBS_ERR_MESH, "tet index out of range");
+  if (flag & 2) return set_error(PBS_ERR_MESH, "rest tet with det(Dm) <= 0 (inverted or degenerate)");
+  return reset_impl<R>(s);
+}
+
+static void free_all(pbs_solver *s) {
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  for (void *q : s->allocs) (void)hipFree(q);
+  s->allocs.clear();
+  for (auto e : s->pool) (void)hipEventDestroy(e);
+  s->pool.clear();
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->st) (void)hipStreamDestroy(s->st);
+  s->st = nullptr;
+}
+
+extern "C" int pbs_abi_version(void) { return PBS_ABI_VERSION; }
+extern "C" const char *pbs_last_error(void) { return g_err.c_str(); }
+
+extern "C" int pbs_default_params(pbs_params *p) {
+  if (!p) return set_error(PBS_ERR_ARG, "null params");
+  std::memset(p, 0, sizeof(*p));
+  p->dt = 0.03f;
+  p->grav[0] = 0.0f;
+  p->grav[1] = -9.8f;
+  p->grav[2] = 0.0f;
+  p->youngs = 1e5f;
+  p->nu = 0.45f;
+  p->density = 1000.0f;
+  p->fixed_mass = 1e8f;
+  p->alpha0 = 0.1f;
+  p->alpha_decay = 0.7f;
+  p->alpha_min = 0.01f;
+  p->hd_refresh = 32;
+  p->energy_check = 8;
+  p->damping = 0.99f;
+  p->init_momentum = 0.2f;
+  p->profile_n = 3;
+  p->profile_k[0] = 1;
+  p->profile_k[1] = 7;
+  p->profile_k[2] = 10;
+  p->profile_rho[0] = 0.96f;
+  p->profile_rho[1] = 0.991f;
+  p->profile_rho[2] = 0.999f;
+  p->semantics = PBS_SEM_REF_GPU;
+  p->precision = PBS_FP32;
+  p->material = PBS_MAT_NEOHOOKEAN;
+  p->iterations = 96;
+  p->relgrad_tol = 0.0;
+  p->max_iterations = 20000;
+  p->block_size = 0;
+  return PBS_OK;
+}
+
+template <typename R> static int set_positions_impl(pbs_solver *s, const float *xyz) {
+  int rc = upload_planes<R>(s, xyz, mesh<R>(s).xpos);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(s->st));
+  return PBS_OK;
+}
+
+extern "C" int pbs_create(int nn, const float *nodes, const float *rest, const int *mask, int nt, const int *tets,
+                          const pbs_params *p, pbs_solver **out) {
+  if (!out) return set_error(PBS_ERR_ARG, "null out");
+  *out = nullptr;
+  if (nn <= 0 || nt < 0 || !rest || (nt > 0 && !tets)) return set_error(PBS_ERR_ARG, "pbs_create: bad mesh arguments");
+  int rc = validate_params(p);
+  if (rc) return rc;
+  for (long i = 0; i < 4L * nt; ++i)
+    if (tets[i] < 0 || tets[i] >= nn) return set_error(PBS_ERR_MESH, "tet index out of range");
+  std::vector<int> off(nn + 1), ent(4 * (size_t)nt);
+  rc = pbs_incidence(nn, nt, tets, off.data(), ent.data());
+  if (rc) return rc;
+  for (auto &e : ent) e /= 3;  // 12*tet + 3*local -> 4*tet + local
+  pbs_solver *s = new pbs_solver();
+  s->nn = nn;
+  s->nt = nt;
+  s->p = *p;
+  make_devparams(s);
+  s->rest_h.assign(rest, rest + 3 * (size_t)nn);
+  auto fail = [&](int code) {
+    free_all(s);
+    delete s;
+    return code;
+  };
+  if (hipGetDevice(&s->device) != hipSuccess) return fail(set_error(PBS_ERR_HIP, "no HIP device"));
+  if (hipStreamCreate(&s->st) != hipSuccess) return fail(set_error(PBS_ERR_HIP, "hipStreamCreate failed"));
+  if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
+    return fail(set_error(PBS_ERR_HIP, "hipEventCreate failed"));
+  rc = p->precision == PBS_FP64 ? alloc_impl<double>(s) : alloc_impl<float>(s);
+  if (rc) return fail(rc);
+  rc = p->precision == PBS_FP64 ? setup_impl<double>(s, tets, mask, off, ent) : setup_impl<float>(s, tets, mask, off, ent);
+  if (rc) return fail(rc);
+  if (nodes) {
+    rc = p->precision == PBS_FP64 ? set_positions_impl<double>(s, nodes) : set_positions_impl<float>(s, nodes);
+    if (rc) return fail(rc);
+  }
+  *out = s;
+  return PBS_OK;
+}
+
+extern "C" void pbs_destroy(pbs_solver *s) {
+  if (!s) return;
+  free_all(s);
+  delete s;
+}
+
+extern "C" int pbs_sizes(const pbs_solver *s, int *nn, int *nt) {
+  if (!s) return set_error(PBS_ERR_ARG, "null handle");
+  if (nn) *nn = s->nn;
+  if (nt) *nt = s->nt;
+  return PBS_OK;
+}
+
+extern "C" int pbs_get_params(const pbs_solver *s, pbs_params *p) {
+  if (!s || !p) return set_error(PBS_ERR_ARG, "null argument");
+  *p = s->p;
+  return PBS_OK;
+}
+
+extern "C" int pbs_set_params(pbs_solver *s, const pbs_params *p) {
+  if (!s) return set_error(PBS_ERR_ARG, "null handle");
+  int rc = validate_params(p);
+  if (rc) return rc;
+  if (p->precision != s->p.precision) return set_error(PBS_ERR_STATE, "precision is fixed at creation");
+  s->p = *p;
+  make_devparams(s);
+  if (s->p.precision == PBS_FP64) {
+    Launch<double>::setup_nodes(s->m64, s->dp, 256, s->st);
+  } else {
+    Launch<float>::setup_nodes(s->m32, s->dp, 256, s->st);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->st));
+  return PBS_OK;
+}
+
+extern "C" int pbs_reset(pbs_solver *s) {
+  if (!s) return set_error(PBS_ERR_ARG, "null handle");
+  return s->p.precision == PBS_FP64 ? reset_impl<double>(s) : reset_impl<float>(s);
+}
+
+extern "C" int pbs_set_positions(pbs_solver *s, const float *xyz) {
+  if (!s || !xyz) return set_error(PBS_ERR_ARG, "null argument");
+  return s->p.precision == PBS_FP64 ? set_positions_impl<double>(s, xyz) : set_positions_impl<float>(s, xyz);
+}
+
+extern "C" int pbs_get_positions(pbs_solver *s, float *xyz) {
+  if (!s || !xyz) return set_error(PBS_ERR_ARG, "null argument");
+  return s->p.precision == PBS_FP64 ? download_planes<double>(s, s->m64.xpos, xyz) : download_planes<float>(s, s->m32.xpos, xyz);
+}
+
+template <typename R> static int get_pos64_impl(pbs_solver *s, double *xyz) {
+  Launch<R>::planes_to_aos64(mesh<R>(s).xpos, s->stage64, s->nn, s->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(xyz, s->stage64, sizeof(double) * 3 * (size_t)s->nn, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return PBS_OK;
+}
+
+extern "C" int pbs_get_positions_f64(pbs_solver *s, double *xyz) {
+  if (!s || !xyz) return set_error(PBS_ERR_ARG, "null argument");
+  return s->p.precision == PBS_FP64 ? get_pos64_impl<double>(s, xyz) : get_pos64_impl<float>(s, xyz);
+}
+
+template <typename R> static int set_vel_impl(pbs_solver *s, const float *v, const float *vl) {
+  int rc = 0;
+  if (v) rc = upload_planes<R>(s, v, mesh<R>(s).v);
+  if (rc) return rc;
+  if (vl) rc = upload_planes<R>(s, vl, mesh<R>(s).vl);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(s->st));
+  return PBS_OK;
+}
+extern "C" int pbs_set_velocities(pbs_solver *s, const float *v, const float *vl) {
+  if (!s) return set_error(PBS_ERR_ARG, "null handle");
+  return s->p.precision == PBS_FP64 ? set_vel_impl<double>(s, v, vl) : set_vel_impl<float>(s, v, vl);
+}
+template <typename R> static int get_vel_impl(pbs_solver *s, float *v, float *vl) {
+  int rc = 0;
+  if (v) rc = download_planes<R>(s, mesh<R>(s).v, v);
+  if (rc) return rc;
+  if (vl) rc = download_planes<R>(s, mesh<R>(s).vl, vl);
+  return rc;
+}
+extern "C" int pbs_get_velocities(pbs_solver *s, float *v, float *vl) {
+  if (!s) return set_error(PBS_ERR_ARG, "null handle");
+  return s->p.precision == PBS_FP64 ? get_vel_impl<double>(s, v, vl) : get_vel_impl<float>(s, v, vl);
+}
+
+// ---- the step ----
+template <typename R> static void launch_pass(pbs_solver *s) {
+  DevMesh<R> &m = mesh<R>(s);
+  int i = timed_begin(s, K_TET);
+  Launch<R>::tet(m, s->dp, s->bs, -1, 0, s->st);
+  timed_end(s, i);
+  i = timed_begin(s, K_NODE);
+  Launch<R>::node(m, s->dp, s->bs, s->st);
+  timed_end(s, i);
+  i = timed_begin(s, K_CONTROL);
+  Launch<R>::control(m, s->dp, s->ntb, s->nnb, s->st);
+  timed_end(s, i);
+}
+
+template <typename R> static int step_impl(pbs_solver *s, const float *ext, float *out, pbs_step_stats *stats) {
+  DevMesh<R> &m = mesh<R>(s);
+  if (ext) {
+    HIPCHK(hipMemcpyAsync(s->stage, ext, sizeof(float) * 3 * (size_t)s->nn, hipMemcpyHostToDevice, s->st));
+    aos_to_planes_f(s->stage, m.ext, s->nn, s->st);
+    s->dp.has_ext = 1;
+  } else {
+    s->dp.has_ext = 0;
+  }
+  HIPCHK(hipEventRecord(s->ev0, s->st));
+  int i = timed_begin(s, K_INIT);
+  Launch<R>::step_init(m, s->dp, s->bs, s->st);
+  timed_end(s, i);
+  HIPCHK(hipGetLastError());
+  const bool relg = s->dp.relgrad_tol > 0;
+  const long cap = relg ? (long)s->p.max_iterations + 256 : (long)s->p.iterations + 256;
+  long launched = 0;
+  int n = relg ? 32 : std::max(s->p.iterations, 1);
+  Ctrl c{};
+  while (true) {
+    for (int k = 0; k < n; ++k) launch_pass<R>(s);
+    HIPCHK(hipGetLastError());
+    launched += n;
+    HIPCHK(hipMemcpyAsync(&c, m.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    if (c.done) break;
+    if (launched >= cap) return set_error(PBS_ERR_STATE, "descent loop did not terminate within the pass cap");
+    n = relg ? 32 : std::max(1, c.budget - c.k) + 1;
+  }
+  i = timed_begin(s, K_FINAL);
+  Launch<R>::step_final(m, s->dp, s->bs, s->st);
+  timed_end(s, i);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(s->ev1, s->st));
+  if (out) {
+    Launch<R>::planes_to_aos(m.xpos, s->stage, s->nn, s->st);
+    HIPCHK(hipMemcpyAsync(out, s->stage, sizeof(float) * 3 * (size_t)s->nn, hipMemcpyDeviceToHost, s->st));
+  }
+  HIPCHK(hipStreamSynchronize(s->st));
+  if (s->profiling) timed_collect(s);
+  if (stats) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
+    stats->passes = c.passes;
+    stats->iterations = c.k;
+    stats->rollbacks = c.rollbacks;
+    stats->converged = c.converged;
+    stats->used_learned = c.used_learned;
+    stats->alpha = c.alpha;
+    stats->energy = c.E_old;
+    stats->g0_norm = c.g0;
+    stats->g_norm = c.g;
+    stats->gpu_ms = ms;
+  }
+  return PBS_OK;
+}
+
+extern "C" int pbs_step(pbs_solver *s, const float *ext, float *out, pbs_step_stats *st) {
+  if (!s) return set_error(PBS_ERR_ARG, "null handle");
+  return s->p.precision == PBS_FP64 ? step_impl<double>(s, ext, out, st) : step_impl<float>(s, ext, out, st);
+}
+
+extern "C" int pbs_do_descent_opt(pbs_solver *s, float h, int iterations, const float *ext, float *out) {
+  (void)h;  // ignored as in the reference (CUDAInterface.cu:911 uses m_timestep)
+  if (!s || !ext || !out || iterations < 0) return set_error(PBS_ERR_ARG, "pbs_do_descent_opt: bad arguments");
+  int saved = s->p.iterations;
+  double saved_tol = s->p.relgrad_tol;
+  s->p.iterations = iterations;
+  s->p.relgrad_tol = 0.0;
+  make_devparams(s);
+  int rc = pbs_step(s, ext, out, nullptr);
+  s->p.iterations = saved;
+  s->p.relgrad_tol = saved_tol;
+  make_devparams(s);
+  return rc;
+}
+
+// ---- material calls ----
+template <typename R> static int download_records(pbs_solver *s, const R *dev, float *host, size_t n) {
+  if (sizeof(R) == sizeof(float)) {
+    HIPCHK(hipMemcpyAsync(host, dev, sizeof(float) * n, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+  } else {
+    std::vector<R> tmp(n);
+    HIPCHK(hipMemcpyAsync(tmp.data(), dev, sizeof(R) * n, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    for (size_t i = 0; i < n; ++i) host[i] = (float)tmp[i];
+  }
+  return PBS_OK;
+}
+
+template <typename R> static int forces_impl(pbs_solver *s, const float *pos, float *tf, float *nf, float *te) {
+  DevMesh<R> &m = mesh<R>(s);
+  int rc = upload_planes<R>(s, pos, m.ring.b[0]);
+  if (rc) return rc;
+  Launch<R>::standalone_ctrl(m, 0, s->st);
+  Launch<R>::tet(m, s->dp, s->bs, 0, 1, s->st);
+  HIPCHK(hipGetLastError());
+  if (nf) {
+    Launch<R>::gather(m, s->dp, s->bs, 0, m.ring.b[1], s->st);
+    rc = download_planes<R>(s, m.ring.b[1], nf);
+    if (rc) return rc;
+  }
+  if (tf && s->nt > 0) {
+    rc = download_records<R>(s, m.frec, tf, 12 * (size_t)s->nt);
+    if (rc) return rc;
+  }
+  if (te && s->nt > 0) {
+    rc = download_records<R>(s, m.tete, te, (size_t)s->nt);
+    if (rc) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(s->st));
+  return PBS_OK;
+}
+
+extern "C" int pbs_compute_forces(pbs_solver *s, const float *pos, float *tf, float *nf, float *te) {
+  if (!s || !pos) return set_error(PBS_ERR_ARG, "null argument");
+  return s->p.precision == PBS_FP64 ? forces_impl<double>(s, pos, tf, nf, te) : forces_impl<float>(s, pos, tf, nf, te);
+}
+
+template <typename R> static int hd_impl(pbs_solver *s, const float *pos, float *nh, float *th) {
+  DevMesh<R> &m = mesh<R>(s);
+  int rc = upload_planes<R>(s, pos, m.ring.b[0]);
+  if (rc) return rc;
+  Launch<R>::standalone_ctrl(m, 0, s->st);
+  Launch<R>::tet(m, s->dp, s->bs, 1, 0, s->st);
+  HIPCHK(hipGetLastError());
+  if (nh) {
+    Launch<R>::gather(m, s->dp, s->bs, 1, m.hd, s->st);
+    planes_to_aos_f(m.hd, s->stage, s->nn, s->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(nh, s->stage, sizeof(float) * 3 * (size_t)s->nn, hipMemcpyDeviceToHost, s->st));
+  }
+  if (th && s->nt > 0) HIPCHK(hipMemcpyAsync(th, m.hrec, sizeof(float) * 12 * (size_t)s->nt, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return PBS_OK;
+}
+
+extern "C" int pbs_compute_hessian_diag(pbs_solver *s, const float *pos, float *nh, float *th) {
+  if (!s || !pos) return set_error(PBS_ERR_ARG, "null argument");
+  return s->p.precision == PBS_FP64 ? hd_impl<double>(s, pos, nh, th) : hd_impl<float>(s, pos, nh, th);
+}
+
+template <typename R> static int rest_data_impl(pbs_solver *s, float *dminv, float *vol, float *mass, float *gforce) {
+  DevMesh<R> &m = mesh<R>(s);
+  const size_t N = s->nn, T = s->nt;
+  if (dminv && T > 0) {
+    std::vector<float> planes(9 * T);
+    HIPCHK(hipMemcpyAsync(planes.data(), m.dmi, sizeof(float) * 9 * T, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    for (size_t t = 0; t < T; ++t)
+      for (int e = 0; e < 9; ++e) dminv[9 * t + e] = planes[e * T + t];
+  }
+  if (vol && T > 0) HIPCHK(hipMemcpyAsync(vol, m.vol, sizeof(float) * T, hipMemcpyDeviceToHost, s->st));
+  if (mass) HIPCHK(hipMemcpyAsync(mass, m.mass, sizeof(float) * N, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  if (gforce) {
+    planes_to_aos_f(m.gforce, s->stage, s->nn, s->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(gforce, s->stage, sizeof(float) * 3 * N, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+  }
+  return PBS_OK;
+}
+
+extern "C" int pbs_get_rest_data(pbs_solver *s, float *dminv, float *vol, float *mass, float *gforce) {
+  if (!s) return set_error(PBS_ERR_ARG, "null handle");
+  return s->p.precision == PBS_FP64 ? rest_data_impl<double>(s, dminv, vol, mass, gforce)
+                                    : rest_data_impl<float>(s, dminv, vol, mass, gforce);
+}
+
+// ---- measurement ----
+extern "C" int pbs_set_profiling(pbs_solver *s, int on) {
+  if (!s) return set_error(PBS_ERR_ARG, "null handle");
+  s->profiling = on != 0;
+  return PBS_OK;
+}
+extern "C" int pbs_reset_kernel_times(pbs_solver *s) {
+  if (!s) return set_error(PBS_ERR_ARG, "null handle");
+  for (int i = 0; i < K_COUNT; ++i) {
+    s->kt_ms[i] = 0;
+    s->kt_n[i] = 0;
+  }
+  return PBS_OK;
+}
+extern "C" int pbs_kernel_times(pbs_solver *s, int max, const char **names, double *avg_ms, long *launches, int *count) {
+  if (!s || !count) return set_error(PBS_ERR_ARG, "null argument");
+  int c = 0;
+  for (int i = 0; i < K_COUNT && c < max; ++i, ++c) {
+    if (names) names[c] = kKernelNames[i];
+    if (avg_ms) avg_ms[c] = s->kt_n[i] ? s->kt_ms[i] / (double)s->kt_n[i] : 0.0;
+    if (launches) launches[c] = s->kt_n[i];
+  }
+  *count = c;
+  return PBS_OK;
+}

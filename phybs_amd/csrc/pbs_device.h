@@ -1,0 +1,290 @@
+// pbs_device.h -- per-tet device math of the descent solver, templated on the real type R
+// (float: the reference's FP32 arithmetic; double: FP64 arithmetic on FP32 mesh data).
+//
+// Conventions (same as the reference's GPU path, CUDAInterface.cu:495-568): 3x3 matrices are
+// row-major; Ds = [x1-x0, x2-x0, x3-x0] (columns); F = Ds Dm^-1; the elastic force on local
+// nodes 1..3 is the matching column of G = P AN and node 0 gets minus their sum.
+// AN = -vol Dm^-T: for a positively oriented rest tet this equals the reference's
+// -(v1 x v2 + v2 x v3 + v3 x v1)/6 area vectors (TetMesh.cpp:279-294), so it is recomputed in
+// registers instead of being stored.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pbs {
+
+template <typename R> __device__ __forceinline__ R dlog(R v);
+template <> __device__ __forceinline__ float dlog<float>(float v) { return logf(v); }
+template <> __device__ __forceinline__ double dlog<double>(double v) { return log(v); }
+template <typename R> __device__ __forceinline__ R drsqrt(R v);
+template <> __device__ __forceinline__ float drsqrt<float>(float v) { return rsqrtf(v); }
+template <> __device__ __forceinline__ double drsqrt<double>(double v) { return rsqrt(v); }
+template <typename R> __device__ __forceinline__ R dsqrt(R v);
+template <> __device__ __forceinline__ float dsqrt<float>(float v) { return sqrtf(v); }
+template <> __device__ __forceinline__ double dsqrt<double>(double v) { return sqrt(v); }
+
+// Adjugate of A (row-major) and det(A) expanded down column 0 (cuMath::Matrix_Inverse_3).
+template <typename R> __device__ __forceinline__ R adj3(const R *A, R *C) {
+  C[0] = A[4] * A[8] - A[7] * A[5];
+  C[1] = A[7] * A[2] - A[1] * A[8];
+  C[2] = A[1] * A[5] - A[4] * A[2];
+  C[3] = A[5] * A[6] - A[3] * A[8];
+  C[4] = A[0] * A[8] - A[2] * A[6];
+  C[5] = A[2] * A[3] - A[0] * A[5];
+  C[6] = A[3] * A[7] - A[4] * A[6];
+  C[7] = A[1] * A[6] - A[0] * A[7];
+  C[8] = A[0] * A[4] - A[1] * A[3];
+  return A[0] * C[0] + A[3] * C[1] + A[6] * C[2];
+}
+
+template <typename R> __device__ __forceinline__ void mul3(const R *A, const R *B, R *C) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
+}
+
+// ---------------------------------------------------------------------------------------------
+// 3x3 SVD after McAdams, Selle, Tamstorf, Teran, Sifakis, "Computing the Singular Value
+// Decomposition of 3x3 matrices with minimal branching and elementary floating point
+// operations", UW-Madison CS TR1690 (2011): Jacobi eigenanalysis of F^T F (here with exact
+// Jacobi rotations and a fixed sweep count), columns of B = F V sorted by norm with sign-
+// preserving swaps (det V = +1), then a Givens QR of B giving U (det U = +1) and the singular
+// values on the diagonal of R; the last one carries the sign of det F.
+// ---------------------------------------------------------------------------------------------
+template <typename R> struct SvdSweeps;
+template <> struct SvdSweeps<float> { static constexpr int n = 5; };
+template <> struct SvdSweeps<double> { static constexpr int n = 8; };
+
+template <typename R> __device__ __forceinline__ void jacobi_rot(R *A, R *V, int p, int q, int r) {
+  // A symmetric stored full row-major; rotate in the (p, q) plane, r is the third index.
+  R apq = A[3 * p + q];
+  R app = A[3 * p + p], aqq = A[3 * q + q];
+  R tiny = (sizeof(R) == 4 ? (R)1e-20 : (R)1e-150);
+  if (fabs(apq) <= tiny * (fabs(app) + fabs(aqq)) + tiny) return;
+  R th = (aqq - app) / ((R)2 * apq);
+  R t = (th >= (R)0 ? (R)1 : (R)-1) / (fabs(th) + dsqrt(th * th + (R)1));
+  R c = drsqrt(t * t + (R)1), s = t * c;
+  A[3 * p + p] = app - t * apq;
+  A[3 * q + q] = aqq + t * apq;
+  A[3 * p + q] = A[3 * q + p] = (R)0;
+  R arp = A[3 * r + p], arq = A[3 * r + q];
+  R nrp = c * arp - s * arq, nrq = s * arp + c * arq;
+  A[3 * r + p] = A[3 * p + r] = nrp;
+  A[3 * r + q] = A[3 * q + r] = nrq;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    R vp = V[3 * i + p], vq = V[3 * i + q];
+    V[3 * i + p] = c * vp - s * vq;
+    V[3 * i + q] = s * vp + c * vq;
+  }
+}
+
+template <typename R> __device__ __forceinline__ void givens_qr_step(R *B, R *U, int p, int q) {
+  // zero B[q][p] using rows p and q (p < q): rows' = G rows, U = U G^T
+  R a = B[3 * p + p], b = B[3 * q + p];
+  R rr = a * a + b * b;
+  R c = (R)1, s = (R)0;
+  if (rr > (R)0) {
+    R inv = drsqrt(rr);
+    c = a * inv;
+    s = b * inv;
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    R bp = B[3 * p + j], bq = B[3 * q + j];
+    B[3 * p + j] = c * bp + s * bq;
+    B[3 * q + j] = -s * bp + c * bq;
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    R up = U[3 * i + p], uq = U[3 * i + q];
+    U[3 * i + p] = c * up + s * uq;
+    U[3 * i + q] = -s * up + c * uq;
+  }
+}
+
+template <typename R> __device__ void svd3(const R *F, R *U, R *S, R *V) {
+  R A[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) A[3 * i + j] = F[i] * F[j] + F[3 + i] * F[3 + j] + F[6 + i] * F[6 + j];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) V[i] = (i % 4 == 0) ? (R)1 : (R)0;
+  for (int sw = 0; sw < SvdSweeps<R>::n; ++sw) {
+    jacobi_rot(A, V, 0, 1, 2);
+    jacobi_rot(A, V, 0, 2, 1);
+    jacobi_rot(A, V, 1, 2, 0);
+  }
+  R B[9];
+  mul3(F, V, B);
+  R n0 = B[0] * B[0] + B[3] * B[3] + B[6] * B[6];
+  R n1 = B[1] * B[1] + B[4] * B[4] + B[7] * B[7];
+  R n2 = B[2] * B[2] + B[5] * B[5] + B[8] * B[8];
+  // sort columns by descending norm; each swap negates one column so det V stays +1
+  auto swapcols = [&](int a, int b) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      R tb = B[3 * i + a];
+      B[3 * i + a] = -B[3 * i + b];
+      B[3 * i + b] = tb;
+      R tv = V[3 * i + a];
+      V[3 * i + a] = -V[3 * i + b];
+      V[3 * i + b] = tv;
+    }
+  };
+  if (n0 < n1) { swapcols(0, 1); R t = n0; n0 = n1; n1 = t; }
+  if (n0 < n2) { swapcols(0, 2); R t = n0; n0 = n2; n2 = t; }
+  if (n1 < n2) { swapcols(1, 2); R t = n1; n1 = n2; n2 = t; }
+  (void)n0; (void)n1; (void)n2;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) U[i] = (i % 4 == 0) ? (R)1 : (R)0;
+  givens_qr_step(B, U, 0, 1);
+  givens_qr_step(B, U, 0, 2);
+  givens_qr_step(B, U, 1, 2);
+  S[0] = B[0];
+  S[1] = B[4];
+  S[2] = B[8];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Per-tet kernels' body: forces, energy and the Hessian (stiffness) diagonal.
+// ---------------------------------------------------------------------------------------------
+// Node-n gradient-of-shape vectors c_n (rows of Dm^-1, node 0 = minus their sum).
+template <typename R> __device__ __forceinline__ void shape_vec(const R *Dmi, int n, R *c) {
+  if (n == 0) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) c[j] = -(Dmi[j] + Dmi[3 + j] + Dmi[6 + j]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) c[j] = Dmi[3 * (n - 1) + j];
+  }
+}
+
+// Neo-Hookean force / energy / stiffness diagonal.
+//   P = mu (F - F^-T) + lambda ln J F^-T                    (CUDAInterface.cu:221-232)
+//   psi = mu/2 (|F|^2 - 3) - mu ln J + lambda/2 (ln J)^2  (CUDAInterface.cu:542-549)
+//   K(n,d) = vol (mu |c_n|^2 + (mu + lambda - lambda ln J) (c_n . w_d)^2), w_d = column d of F^-1:
+//   the diagonal of c_n^T (dP_d./dF_d.) c_n for the closed-form derivative
+//   dP_ij/dF_kl = mu d_ik d_jl + (mu - lambda ln J) Finv_li Finv_jk + lambda Finv_ji Finv_lk,
+//   which is what the reference's SVD/Xu-2015 route (CUDAInterface.cu:101-219) evaluates.
+template <typename R, bool HD>
+__device__ __forceinline__ void neo_tet(const R *F, const R *Dmi, R vol, R mu, R lam, R *f12, R &energy, R *h12) {
+  R C[9];
+  R J = adj3(F, C);
+  R iJ = (R)1 / J;
+  R Fi[9];  // F^-1 = C / J
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Fi[i] = C[i] * iJ;
+  R lj = dlog(J);
+  R P[9];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      R fit = Fi[3 * c + a];  // (F^-T)[a][c]
+      P[3 * a + c] = (F[3 * a + c] - fit) * mu + lam * lj * fit;
+    }
+  // G = P AN = -vol P Dm^-T: G[a][b] = -vol sum_c P[a][c] Dmi[b][c]
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    R g0 = -vol * (P[3 * a] * Dmi[0] + P[3 * a + 1] * Dmi[1] + P[3 * a + 2] * Dmi[2]);
+    R g1 = -vol * (P[3 * a] * Dmi[3] + P[3 * a + 1] * Dmi[4] + P[3 * a + 2] * Dmi[5]);
+    R g2 = -vol * (P[3 * a] * Dmi[6] + P[3 * a + 1] * Dmi[7] + P[3 * a + 2] * Dmi[8]);
+    f12[3 + a] = g0;
+    f12[6 + a] = g1;
+    f12[9 + a] = g2;
+    f12[a] = -(g0 + g1 + g2);
+  }
+  R I1 = (R)0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) I1 += F[i] * F[i];
+  energy = ((R)0.5 * mu * (I1 - (R)3) - mu * lj + (R)0.5 * lam * lj * lj) * vol;
+  if (HD) {
+    R k2 = mu + lam - lam * lj;
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      R c[3];
+      shape_vec(Dmi, n, c);
+      R cc = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        R s = c[0] * Fi[d] + c[1] * Fi[3 + d] + c[2] * Fi[6 + d];
+        h12[3 * n + d] = vol * (mu * cc + k2 * s * s);
+      }
+    }
+  }
+}
+
+// Stretch material psi = sum mu (s_i - 1)^2 + lambda/2 (J - 1)^2 on signed singular values.
+//   P = 2 mu (F - U V^T) + lambda (J - 1) J F^-T.
+//   K(n,d) = vol [ 2 mu (|c_n|^2 - sum_{k<l} (a_kl . c_n)^2 / (s_k + s_l)) + lambda J^2 (c_n . w_d)^2 ]
+//   with a_kl[c] = U_dk V_cl - U_dl V_ck (the rotation part of Xu et al. 2015's 2x2 solves,
+//   written in closed form).
+template <typename R, bool HD>
+__device__ __forceinline__ void stretch_tet(const R *F, const R *Dmi, R vol, R mu, R lam, R *f12, R &energy,
+                                            R *h12) {
+  R U[9], S[3], V[9];
+  svd3(F, U, S, V);
+  R C[9];
+  R J = adj3(F, C);
+  R iJ = (R)1 / J;
+  R Fi[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Fi[i] = C[i] * iJ;
+  R Rot[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Rot[3 * i + j] = U[3 * i] * V[3 * j] + U[3 * i + 1] * V[3 * j + 1] + U[3 * i + 2] * V[3 * j + 2];
+  R cj = lam * (J - (R)1) * J;
+  R P[9];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) P[3 * a + c] = (R)2 * mu * (F[3 * a + c] - Rot[3 * a + c]) + cj * Fi[3 * c + a];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    R g0 = -vol * (P[3 * a] * Dmi[0] + P[3 * a + 1] * Dmi[1] + P[3 * a + 2] * Dmi[2]);
+    R g1 = -vol * (P[3 * a] * Dmi[3] + P[3 * a + 1] * Dmi[4] + P[3 * a + 2] * Dmi[5]);
+    R g2 = -vol * (P[3 * a] * Dmi[6] + P[3 * a + 1] * Dmi[7] + P[3 * a + 2] * Dmi[8]);
+    f12[3 + a] = g0;
+    f12[6 + a] = g1;
+    f12[9 + a] = g2;
+    f12[a] = -(g0 + g1 + g2);
+  }
+  R e = (S[0] - (R)1) * (S[0] - (R)1) + (S[1] - (R)1) * (S[1] - (R)1) + (S[2] - (R)1) * (S[2] - (R)1);
+  energy = (mu * e + (R)0.5 * lam * (J - (R)1) * (J - (R)1)) * vol;
+  if (HD) {
+    const int kk[3] = {0, 0, 1}, ll[3] = {1, 2, 2};
+    R inv[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      R den = S[kk[q]] + S[ll[q]];
+      R lim = (sizeof(R) == 4 ? (R)1e-6 : (R)1e-12);
+      inv[q] = (R)1 / (den > lim ? den : lim);
+    }
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      R c[3];
+      shape_vec(Dmi, n, c);
+      R cc = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        R rot = (R)0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          int k = kk[q], l = ll[q];
+          R ac = (U[3 * d + k] * V[l] - U[3 * d + l] * V[k]) * c[0] +
+                 (U[3 * d + k] * V[3 + l] - U[3 * d + l] * V[3 + k]) * c[1] +
+                 (U[3 * d + k] * V[6 + l] - U[3 * d + l] * V[6 + k]) * c[2];
+          rot += ac * ac * inv[q];
+        }
+        R s = c[0] * Fi[d] + c[1] * Fi[3 + d] + c[2] * Fi[6 + d];
+        h12[3 * n + d] = vol * ((R)2 * mu * (cc - rot) + lam * J * J * s * s);
+      }
+    }
+  }
+}
+
+}  // namespace pbs

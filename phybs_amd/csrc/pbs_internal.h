@@ -1,0 +1,91 @@
+// pbs_internal.h -- structures shared by the kernels (pbs_kernels.hip) and the host driver
+// (pbs_capi.cpp).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pbs {
+
+// Kernel-side copy of the solver constants (passed by value to every kernel).
+struct DevParams {
+  int nn, nt;
+  float h, mu, lam, density, fixed_mass;
+  float grav[3];
+  float alpha0, alpha_decay, alpha_min;
+  int hd_refresh, energy_check;
+  float damping, init_momentum;
+  int profile_n;
+  int profile_k[8];
+  float profile_rho[8];
+  int semantics;  // 0 REF_GPU, 1 REF_CPU
+  int material;   // 0 neo-Hookean, 1 stretch
+  int clamp;      // 1 max(E,0), 2 E>1e-6?E:0
+  float ext_scale;
+  int has_ext;
+  int iterations;
+  double relgrad_tol;
+};
+
+// Device-resident loop state.  The reference keeps k, alpha, the energies and the Chebyshev
+// scalars on the host and copies node arrays to roll back (CUDAInterface.cu:911-1003); here
+// the control kernel decides on the device and rollbacks/swaps are buffer-role changes.
+struct Ctrl {
+  int k, budget, done, passes;
+  int rollbacks, converged, used_learned, first;
+  int cur, nxt, last, old;  // roles of the 4 position buffers
+  int hd_now, check_now, cheb_now, kc;  // kc: iteration of the last Chebyshev restart
+  float alpha, rho, omega, rho_next, omega_next, pad2;
+  double E, E_old, g0, g;
+};
+
+template <typename R> struct Ring {
+  R *b[4];  // each: 3 planes of nn values (x | y | z)
+};
+
+// Device buffers of one solver instance.
+template <typename R> struct DevMesh {
+  int4 *tets;       // T
+  float *rest;      // 3 planes
+  float *dmi;       // 9 planes of T (Dm^-1 row-major entry e at dmi[e*T + t])
+  float *vol;       // T
+  int *inc_off;     // N+1
+  int *inc;         // 4T, entry = 4*tet + local
+  int *mask;        // N
+  float *mass;      // N (fixed nodes: fixed_mass)
+  float *gforce;    // 3 planes: mass (before the fixed override) * grav
+  float *ext;       // 3 planes
+  float *hd;        // 3 planes: Hessian diagonal
+  R *frec;          // 12T per-tet force records
+  float *hrec;      // 12T per-tet Hessian-diagonal records
+  R *xt;            // 3 planes: inertial target x~
+  R *xpos;          // 3 planes: positions at step start / after the step
+  R *v, *vl;        // 3 planes each
+  Ring<R> ring;
+  double *etet;     // per tet-block energy partials
+  double *enode;    // per node-block energy partials
+  double *gpart;    // per node-block |g|^2 partials
+  Ctrl *ctrl;
+  int *flag;        // setup error flag
+  R *tete;          // T, optional per-tet energy output
+};
+
+// ---- launch wrappers (pbs_kernels.hip) ----
+template <typename R> struct Launch {
+  static void setup_tets(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st);
+  static void setup_nodes(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st);
+  static void tet(const DevMesh<R> &m, const DevParams &p, int bs, int force_hd, int write_e, hipStream_t st);
+  static void node(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st);
+  static void control(const DevMesh<R> &m, const DevParams &p, int ntb, int nnb, hipStream_t st);
+  static void step_init(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st);
+  static void step_final(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st);
+  static void standalone_ctrl(const DevMesh<R> &m, int cur, hipStream_t st);
+  static void gather(const DevMesh<R> &m, const DevParams &p, int bs, int which, void *out_planes, hipStream_t st);
+  static void aos_to_planes(const float *aos, R *planes, int n, hipStream_t st);
+  static void planes_to_aos(const R *planes, float *aos, int n, hipStream_t st);
+  static void planes_to_aos64(const R *planes, double *aos, int n, hipStream_t st);
+};
+void aos_to_planes_f(const float *aos, float *planes, int n, hipStream_t st);
+void planes_to_aos_f(const float *planes, float *aos, int n, hipStream_t st);
+void fill_f(float *p, float v, long n, hipStream_t st);
+
+}  // namespace pbs

@@ -1,0 +1,558 @@
+// pbs_kernels.hip -- HIP kernels of the descent solver for gfx950 (CDNA4, wave64).
+//
+// One descent pass = k_tet (per-tet force, energy, and on refresh passes the Hessian
+// diagonal) -> k_node (atomic-free gather over the tet-incidence CSR, Jacobi step, Chebyshev
+// update, energy and |g|^2 partials) -> k_control (FP64 reduction of the partials and the
+// line-search / Chebyshev / buffer-role decisions on the device).  This replaces the
+// reference's computeForceKd / updateNodesNextIter / updateChebyshev / thrust::reduce and the
+// host loop of CUDAInterface::doDescentOpt (CUDAInterface.cu:482-1007) with no host round trip.
+//
+// Layout: node vectors are 3 planes (x | y | z) of nn values (SoA); per-tet rest data are 10
+// planes of nt floats (Dm^-1 entries, volume); per-tet force records are 12 consecutive values
+// (4 local nodes x xyz) so that a node's gather reads 12 contiguous bytes per incident tet.
+#include "pbs_device.h"
+#include "pbs_internal.h"
+
+namespace pbs {
+
+// ---------------------------------------------------------------------------------------------
+// Deterministic block reduction of a double: wave64 shuffle tree, then the waves' sums in order.
+// The result is valid in thread 0.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) lds[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) r += lds[i];
+  }
+  __syncthreads();
+  return r;
+}
+
+// Chebyshev schedule of the reference (CUDAInterface.cu:967-986; DescentOptimize.cpp:156-175).
+__device__ __forceinline__ void cheb_schedule(const DevParams &p, int k, float &rho, float &omega) {
+  if (k == 0) {
+    rho = 0.0f;
+    omega = 1.0f;
+  }
+  omega = 4.0f / (4.0f - rho * rho * omega);
+  for (int i = 0; i < p.profile_n; ++i) {
+    if (k == p.profile_k[i] - 1) {
+      rho = 0.0f;
+      omega = 1.0f;
+    }
+    if (k == p.profile_k[i]) {
+      rho = p.profile_rho[i];
+      omega = 2.0f / (2.0f - rho * rho);
+      break;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Setup: rest-shape data (compute_DmInv_ANs_Mass, CUDAInterface.cu:273-325; TetMesh.cpp:189-216).
+// Contraction is off so that Dm^-1, volumes and masses round exactly as the FP32 reference
+// formulas written out without fused multiply-adds.
+// ---------------------------------------------------------------------------------------------
+#pragma clang fp contract(off)
+__global__ void k_setup_tets(DevParams p, const float *__restrict__ rest, const int4 *__restrict__ tets,
+                             float *__restrict__ dmi, float *__restrict__ vol, int *__restrict__ flag) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= p.nt) return;
+  const int N = p.nn, T = p.nt;
+  const int4 q = tets[t];
+  if (q.x < 0 || q.y < 0 || q.z < 0 || q.w < 0 || q.x >= N || q.y >= N || q.z >= N || q.w >= N) {
+    atomicOr(flag, 1);
+    return;
+  }
+  float Dm[9], C[9];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float x0 = rest[k * N + q.x];
+    Dm[3 * k + 0] = rest[k * N + q.y] - x0;
+    Dm[3 * k + 1] = rest[k * N + q.z] - x0;
+    Dm[3 * k + 2] = rest[k * N + q.w] - x0;
+  }
+  // adjugate and determinant written out here so that the no-contraction pragma applies
+  C[0] = Dm[4] * Dm[8] - Dm[7] * Dm[5];
+  C[1] = Dm[7] * Dm[2] - Dm[1] * Dm[8];
+  C[2] = Dm[1] * Dm[5] - Dm[4] * Dm[2];
+  C[3] = Dm[5] * Dm[6] - Dm[3] * Dm[8];
+  C[4] = Dm[0] * Dm[8] - Dm[2] * Dm[6];
+  C[5] = Dm[2] * Dm[3] - Dm[0] * Dm[5];
+  C[6] = Dm[3] * Dm[7] - Dm[4] * Dm[6];
+  C[7] = Dm[1] * Dm[6] - Dm[0] * Dm[7];
+  C[8] = Dm[0] * Dm[4] - Dm[1] * Dm[3];
+  const float det = Dm[0] * C[0] + Dm[3] * C[1] + Dm[6] * C[2];
+  if (!(det > 0.0f)) atomicOr(flag, 2);
+  vol[t] = fabsf(det) / 6.0f;
+  const float inv = 1.0f / det;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) dmi[(size_t)i * T + t] = C[i] * inv;
+}
+
+// Lumped mass gathered over the incidence list in ascending tet order (the reference's float
+// atomics, CUDAInterface.cu:297-301, sum in an unspecified order); gravity force from the mass
+// before constrained nodes get fixed_mass (CUDAInterface.cu:716-719).
+__global__ void k_setup_nodes(DevParams p, const int *__restrict__ inc_off, const int *__restrict__ inc,
+                              const float *__restrict__ vol, const int *__restrict__ mask, float *__restrict__ mass,
+                              float *__restrict__ gforce) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= p.nn) return;
+  float m = 0.0f;
+  for (int e = inc_off[n]; e < inc_off[n + 1]; ++e) {
+    const float w = vol[inc[e] >> 2] / 4.0f * p.density;
+    m += w;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) gforce[(size_t)d * p.nn + n] = p.grav[d] * m;
+  mass[n] = mask[n] ? p.fixed_mass : m;
+}
+#pragma clang fp contract(on)
+
+// ---------------------------------------------------------------------------------------------
+// Per-tet pass: forces, energy, optional Hessian diagonal (computeForceKd, CUDAInterface.cu:495-568).
+// One thread per tet; node positions are gathered from the 3 planes, rest data read coalesced.
+// ---------------------------------------------------------------------------------------------
+template <typename R, int MAT>
+__global__ void __launch_bounds__(256) k_tet(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring,
+                                             const int4 *__restrict__ tets, const float *__restrict__ dmi,
+                                             const float *__restrict__ vol, R *__restrict__ frec,
+                                             float *__restrict__ hrec, double *__restrict__ etet,
+                                             R *__restrict__ tete, int force_hd) {
+  __shared__ double lds[16];
+  if (ctrl->done) return;
+  const int hd = force_hd >= 0 ? force_hd : ctrl->hd_now;
+  const R *__restrict__ x = ring.b[ctrl->cur];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = p.nn, T = p.nt;
+  double e = 0.0;
+  if (t < T) {
+    const int4 q = tets[t];
+    R Dmi[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Dmi[i] = (R)dmi[(size_t)i * T + t];
+    const R V = (R)vol[t];
+    R Ds[9];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const R *xk = x + (size_t)k * N;
+      const R x0 = xk[q.x];
+      Ds[3 * k + 0] = xk[q.y] - x0;
+      Ds[3 * k + 1] = xk[q.z] - x0;
+      Ds[3 * k + 2] = xk[q.w] - x0;
+    }
+    R F[9];
+    mul3(Ds, Dmi, F);
+    R f12[12], h12[12], en;
+    const R mu = (R)p.mu, lam = (R)p.lam;
+    if (hd) {
+      if (MAT == 0) neo_tet<R, true>(F, Dmi, V, mu, lam, f12, en, h12);
+      else stretch_tet<R, true>(F, Dmi, V, mu, lam, f12, en, h12);
+      float *hr = hrec + 12 * (size_t)t;
+#pragma unroll
+      for (int i = 0; i < 12; ++i) hr[i] = (float)h12[i];
+    } else {
+      if (MAT == 0) neo_tet<R, false>(F, Dmi, V, mu, lam, f12, en, h12);
+      else stretch_tet<R, false>(F, Dmi, V, mu, lam, f12, en, h12);
+    }
+    R *fr = frec + 12 * (size_t)t;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) fr[i] = f12[i];
+    if (p.clamp == 1) en = en > (R)0 ? en : (R)0;                  // CUDAInterface.cu:549
+    else if (p.clamp == 2) en = en > (R)1e-6 ? en : (R)0;          // IsotropicNeohookeanMaterial.cpp:129
+    if (tete) tete[t] = en;
+    e = (double)en;
+  }
+  const double s = block_sum(e, lds);
+  if (threadIdx.x == 0) etet[blockIdx.x] = s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Per-node pass: gather + Jacobi-preconditioned step + Chebyshev (updateNodesNextIter and
+// updateChebyshev, CUDAInterface.cu:572-623).  g = (x - x~) m/h^2 - f - s (m grav + f_ext)
+// with s = 1 (REF_GPU) or 2 (REF_CPU, DescentOptimize.cpp:51-52); fixed nodes keep x.
+// ---------------------------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(256) k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring,
+                                              const int *__restrict__ inc_off, const int *__restrict__ inc,
+                                              const R *__restrict__ frec, const float *__restrict__ hrec,
+                                              float *__restrict__ hd, const float *__restrict__ mass,
+                                              const int *__restrict__ mask, const float *__restrict__ gforce,
+                                              const float *__restrict__ ext, const R *__restrict__ xt,
+                                              double *__restrict__ enode, double *__restrict__ gpart) {
+  __shared__ double lds[16];
+  if (ctrl->done) return;
+  const int hdnow = ctrl->hd_now, cheb = ctrl->cheb_now;
+  const R alpha = (R)ctrl->alpha, om = (R)ctrl->omega_next;
+  const R *__restrict__ x = ring.b[ctrl->cur];
+  const R *__restrict__ xl = ring.b[ctrl->last];
+  R *__restrict__ xn = ring.b[ctrl->nxt];
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = p.nn;
+  double en = 0.0, g2 = 0.0;
+  if (n < N) {
+    const int b = inc_off[n], end = inc_off[n + 1];
+    R f0 = 0, f1 = 0, f2 = 0;
+    for (int e = b; e < end; ++e) {
+      const R *r = frec + 3 * (size_t)inc[e];
+      f0 += r[0];
+      f1 += r[1];
+      f2 += r[2];
+    }
+    float h0, h1, h2;
+    if (hdnow) {
+      h0 = h1 = h2 = 0.0f;
+      for (int e = b; e < end; ++e) {
+        const float *r = hrec + 3 * (size_t)inc[e];
+        h0 += r[0];
+        h1 += r[1];
+        h2 += r[2];
+      }
+      hd[n] = h0;
+      hd[N + n] = h1;
+      hd[2 * (size_t)N + n] = h2;
+    } else {
+      h0 = hd[n];
+      h1 = hd[N + n];
+      h2 = hd[2 * (size_t)N + n];
+    }
+    const R fx[3] = {f0, f1, f2};
+    const float hh[3] = {h0, h1, h2};
+    if (mask[n]) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) xn[(size_t)d * N + n] = x[(size_t)d * N + n];
+    } else {
+      const R hstep = (R)p.h;
+      const R mh2 = (R)mass[n] / (hstep * hstep);
+      R e = (R)0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const size_t i = (size_t)d * N + n;
+        const R xc = x[i];
+        const R dd = xc - xt[i];
+        R fe = (R)gforce[i];
+        if (p.has_ext) fe += (R)ext[i];
+        const R g = dd * mh2 - fx[d] - (R)p.ext_scale * fe;
+        e += (R)0.5 * dd * dd * mh2 - fe * xc;
+        g2 += (double)g * (double)g;
+        R nx = xc - alpha * g / ((R)hh[d] + mh2);
+        if (cheb) {
+          const R l = xl[i];
+          nx = om * (nx - l) + l;
+        }
+        xn[i] = nx;
+      }
+      en = (double)e;
+    }
+  }
+  const double se = block_sum(en, lds);
+  const double sg = block_sum(g2, lds);
+  if (threadIdx.x == 0) {
+    enode[blockIdx.x] = se;
+    gpart[blockIdx.x] = sg;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Control: FP64 sums of the partials in a fixed order, then the reference's loop logic
+// (CUDAInterface.cu:917-1003) on the device.
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_control(DevParams p, Ctrl *__restrict__ ctrl,
+                                                 const double *__restrict__ etet, int ntb,
+                                                 const double *__restrict__ enode,
+                                                 const double *__restrict__ gpart, int nnb) {
+  __shared__ double la[256], lb[256], lc[256];
+  if (ctrl->done) return;
+  const int tid = threadIdx.x;
+  const int check = ctrl->check_now;
+  double a = 0.0, b = 0.0, c = 0.0;
+  if (check)
+    for (int i = tid; i < ntb; i += 256) a += etet[i];
+  for (int i = tid; i < nnb; i += 256) {
+    if (check) b += enode[i];
+    c += gpart[i];
+  }
+  la[tid] = a;
+  lb[tid] = b;
+  lc[tid] = c;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      la[tid] += la[tid + s];
+      lb[tid] += lb[tid + s];
+      lc[tid] += lc[tid + s];
+    }
+    __syncthreads();
+  }
+  if (tid != 0) return;
+  Ctrl C = *ctrl;
+  const double g = sqrt(lc[0]);
+  C.g = g;
+  if (C.first) {
+    if (C.g0 < 0.0) C.g0 = g;
+    C.first = 0;
+  }
+  if (p.relgrad_tol > 0.0 && g <= p.relgrad_tol * C.g0) {
+    C.converged = 1;
+    C.done = 1;
+    C.passes++;
+    *ctrl = C;
+    return;
+  }
+  int rb = 0;
+  const bool robust = p.relgrad_tol > 0.0;
+  if (check) {
+    const double E = la[0] + lb[0];
+    C.E = E;
+    // Rel-grad mode (no reference counterpart) also rejects a non-finite energy.
+    const bool worse = robust ? !(E <= C.E_old) : (E > C.E_old);
+    if (C.k != 0 && worse) {  // rollback (CUDAInterface.cu:946-958)
+      C.alpha *= p.alpha_decay;
+      if (robust) C.kc = C.k;  // rel-grad mode: keep the budget, restart Chebyshev at k
+      else C.budget -= C.k - p.energy_check;
+      C.cur = C.old;
+      C.rollbacks++;
+      rb = 1;
+    } else {
+      C.E_old = E;
+      C.old = C.cur;
+    }
+  }
+  if (!rb) {
+    C.rho = C.rho_next;
+    C.omega = C.omega_next;
+    C.last = C.cur;
+    C.cur = C.nxt;
+    int f = 0;
+    while (f == C.cur || f == C.last || f == C.old) ++f;
+    C.nxt = f;
+    C.k++;
+  }
+  C.passes++;
+  if (C.alpha < p.alpha_min || !(C.k < C.budget)) {
+    C.done = 1;
+  } else {
+    C.hd_now = (C.k % p.hd_refresh) == 0;
+    C.check_now = (C.k % p.energy_check) == 0;
+    float rho = C.rho, om = C.omega;
+    cheb_schedule(p, C.k - C.kc, rho, om);
+    C.rho_next = rho;
+    C.omega_next = om;
+    C.cheb_now = om != 1.0f;
+  }
+  *ctrl = C;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Step start (initDescentOpt, CUDAInterface.cu:482-493; DescentOptimize.cpp:35-41) and the
+// device loop state reset.
+// ---------------------------------------------------------------------------------------------
+template <typename R>
+__global__ void k_step_init(DevParams p, Ctrl *__restrict__ ctrl, Ring<R> ring, const R *__restrict__ xpos,
+                            const R *__restrict__ v, const R *__restrict__ vl, R *__restrict__ xt) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < p.nn) {
+    const R h = (R)p.h;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const size_t i = (size_t)d * p.nn + n;
+      const R t = xpos[i] + v[i] * h;
+      xt[i] = t;
+      R x0;
+      if (p.semantics == 1) x0 = t + (R)(p.init_momentum * p.h) * (v[i] - vl[i]);
+      else if (sizeof(R) == 4) x0 = (R)((double)t + (double)p.init_momentum * (double)(v[i] - vl[i]) * (double)p.h);
+      else x0 = t + (R)p.init_momentum * (v[i] - vl[i]) * h;
+      ring.b[0][i] = x0;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    Ctrl C;
+    C.k = 0;
+    C.budget = p.iterations;
+    C.passes = 0;
+    C.rollbacks = 0;
+    C.converged = 0;
+    C.used_learned = 0;
+    C.first = 1;
+    C.cur = 0;
+    C.nxt = 1;
+    C.last = 2;
+    C.old = 3;
+    C.hd_now = 1;
+    C.check_now = 1;
+    C.kc = 0;
+    C.alpha = p.alpha0;
+    C.rho = 0.0f;
+    C.omega = 1.0f;
+    float rho = 0.0f, om = 1.0f;
+    cheb_schedule(p, 0, rho, om);
+    C.rho_next = rho;
+    C.omega_next = om;
+    C.cheb_now = om != 1.0f;
+    C.pad2 = 0.0f;
+    C.E = 0.0;
+    C.E_old = 0.0;
+    C.g0 = -1.0;
+    C.g = 0.0;
+    C.done = (p.iterations <= 0 || p.alpha0 < p.alpha_min) ? 1 : 0;
+    *ctrl = C;
+  }
+}
+
+// Step end: velocities (updateVelocity, CUDAInterface.cu:625-639, or DescentOptimize.cpp:183-185)
+// and the new positions.
+template <typename R>
+__global__ void k_step_final(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, R *__restrict__ xpos,
+                             R *__restrict__ v, R *__restrict__ vl, const R *__restrict__ xt,
+                             const int *__restrict__ mask) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= p.nn) return;
+  const R *__restrict__ x = ring.b[ctrl->cur];
+  const R h = (R)p.h;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const size_t i = (size_t)d * p.nn + n;
+    const R xf = x[i];
+    if (p.semantics == 1) {
+      vl[i] = v[i];
+      v[i] = (R)p.damping * (xf - xpos[i]) / h;
+    } else if (!mask[n]) {
+      vl[i] = v[i];
+      const R vv = v[i] + (xf - xt[i]) * ((R)1 / h);
+      v[i] = (sizeof(R) == 4) ? (R)((double)vv * (double)p.damping) : vv * (R)p.damping;
+    }
+    xpos[i] = xf;
+  }
+}
+
+// Control block for a standalone force / Hessian evaluation on ring buffer `cur`.
+__global__ void k_standalone_ctrl(Ctrl *ctrl, int cur) {
+  Ctrl C = {};
+  C.cur = cur;
+  C.nxt = (cur + 1) & 3;
+  C.last = (cur + 2) & 3;
+  C.old = (cur + 3) & 3;
+  C.done = 0;
+  *ctrl = C;
+}
+
+// Gather of per-tet records into per-node planes (elastic forces or Hessian diagonal).
+template <typename T>
+__global__ void k_gather(int N, const int *__restrict__ inc_off, const int *__restrict__ inc,
+                         const T *__restrict__ rec, T *__restrict__ out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  T a = 0, b = 0, c = 0;
+  for (int e = inc_off[n]; e < inc_off[n + 1]; ++e) {
+    const T *r = rec + 3 * (size_t)inc[e];
+    a += r[0];
+    b += r[1];
+    c += r[2];
+  }
+  out[n] = a;
+  out[N + n] = b;
+  out[2 * (size_t)N + n] = c;
+}
+
+template <typename A, typename B>
+__global__ void k_aos_to_planes(const A *__restrict__ aos, B *__restrict__ planes, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) planes[(size_t)d * n + i] = (B)aos[3 * (size_t)i + d];
+}
+
+template <typename A, typename B>
+__global__ void k_planes_to_aos(const A *__restrict__ planes, B *__restrict__ aos, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) aos[3 * (size_t)i + d] = (B)planes[(size_t)d * n + i];
+}
+
+__global__ void k_fill(float *p, float v, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Launch wrappers
+// ---------------------------------------------------------------------------------------------
+static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+template <typename R>
+void Launch<R>::setup_tets(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st) {
+  if (p.nt > 0) hipLaunchKernelGGL(k_setup_tets, dim3(nblk(p.nt, bs)), dim3(bs), 0, st, p, m.rest, m.tets, m.dmi, m.vol, m.flag);
+}
+template <typename R>
+void Launch<R>::setup_nodes(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st) {
+  hipLaunchKernelGGL(k_setup_nodes, dim3(nblk(p.nn, bs)), dim3(bs), 0, st, p, m.inc_off, m.inc, m.vol, m.mask, m.mass, m.gforce);
+}
+template <typename R>
+void Launch<R>::tet(const DevMesh<R> &m, const DevParams &p, int bs, int force_hd, int write_e, hipStream_t st) {
+  if (p.nt <= 0) return;
+  R *te = write_e ? m.tete : nullptr;
+  if (p.material == 0)
+    hipLaunchKernelGGL((k_tet<R, 0>), dim3(nblk(p.nt, bs)), dim3(bs), 0, st, p, m.ctrl, m.ring, m.tets, m.dmi, m.vol, m.frec, m.hrec, m.etet, te, force_hd);
+  else
+    hipLaunchKernelGGL((k_tet<R, 1>), dim3(nblk(p.nt, bs)), dim3(bs), 0, st, p, m.ctrl, m.ring, m.tets, m.dmi, m.vol, m.frec, m.hrec, m.etet, te, force_hd);
+}
+template <typename R>
+void Launch<R>::node(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st) {
+  hipLaunchKernelGGL((k_node<R>), dim3(nblk(p.nn, bs)), dim3(bs), 0, st, p, m.ctrl, m.ring, m.inc_off, m.inc, m.frec, m.hrec, m.hd, m.mass, m.mask, m.gforce, m.ext, m.xt, m.enode, m.gpart);
+}
+template <typename R>
+void Launch<R>::control(const DevMesh<R> &m, const DevParams &p, int ntb, int nnb, hipStream_t st) {
+  hipLaunchKernelGGL(k_control, dim3(1), dim3(256), 0, st, p, m.ctrl, m.etet, ntb, m.enode, m.gpart, nnb);
+}
+template <typename R>
+void Launch<R>::step_init(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st) {
+  hipLaunchKernelGGL((k_step_init<R>), dim3(nblk(p.nn, bs)), dim3(bs), 0, st, p, m.ctrl, m.ring, m.xpos, m.v, m.vl, m.xt);
+}
+template <typename R>
+void Launch<R>::step_final(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st) {
+  hipLaunchKernelGGL((k_step_final<R>), dim3(nblk(p.nn, bs)), dim3(bs), 0, st, p, m.ctrl, m.ring, m.xpos, m.v, m.vl, m.xt, m.mask);
+}
+template <typename R>
+void Launch<R>::standalone_ctrl(const DevMesh<R> &m, int cur, hipStream_t st) {
+  hipLaunchKernelGGL(k_standalone_ctrl, dim3(1), dim3(1), 0, st, m.ctrl, cur);
+}
+template <typename R>
+void Launch<R>::gather(const DevMesh<R> &m, const DevParams &p, int bs, int which, void *out, hipStream_t st) {
+  if (which == 0)
+    hipLaunchKernelGGL((k_gather<R>), dim3(nblk(p.nn, bs)), dim3(bs), 0, st, p.nn, m.inc_off, m.inc, m.frec, (R *)out);
+  else
+    hipLaunchKernelGGL((k_gather<float>), dim3(nblk(p.nn, bs)), dim3(bs), 0, st, p.nn, m.inc_off, m.inc, m.hrec, (float *)out);
+}
+template <typename R>
+void Launch<R>::aos_to_planes(const float *aos, R *planes, int n, hipStream_t st) {
+  hipLaunchKernelGGL((k_aos_to_planes<float, R>), dim3(nblk(n, 256)), dim3(256), 0, st, aos, planes, n);
+}
+template <typename R>
+void Launch<R>::planes_to_aos(const R *planes, float *aos, int n, hipStream_t st) {
+  hipLaunchKernelGGL((k_planes_to_aos<R, float>), dim3(nblk(n, 256)), dim3(256), 0, st, planes, aos, n);
+}
+template <typename R>
+void Launch<R>::planes_to_aos64(const R *planes, double *aos, int n, hipStream_t st) {
+  hipLaunchKernelGGL((k_planes_to_aos<R, double>), dim3(nblk(n, 256)), dim3(256), 0, st, planes, aos, n);
+}
+void aos_to_planes_f(const float *aos, float *planes, int n, hipStream_t st) {
+  hipLaunchKernelGGL((k_aos_to_planes<float, float>), dim3(nblk(n, 256)), dim3(256), 0, st, aos, planes, n);
+}
+void planes_to_aos_f(const float *planes, float *aos, int n, hipStream_t st) {
+  hipLaunchKernelGGL((k_planes_to_aos<float, float>), dim3(nblk(n, 256)), dim3(256), 0, st, planes, aos, n);
+}
+void fill_f(float *p, float v, long n, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, st, p, v, n);
+}
+
+template struct Launch<float>;
+template struct Launch<double>;
+
+}  // namespace pbs

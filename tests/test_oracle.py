@@ -1,0 +1,394 @@
+"""Pins of the CPU oracle (oracle/pbs_oracle.c) against what the mathematics fixes.
+
+The reference ships no golden vectors and its CPU path does not build here, so every oracle
+function is checked against closed forms, invariants, central finite differences in FP64, or an
+independent numpy / scipy computation (never a retyping of the oracle's own formula).
+"""
+import os
+
+import numpy as np
+import pytest
+from scipy.optimize import minimize
+
+import meshgen as M
+from oracle import pyoracle as O
+
+E_, NU, RHO = 1e5, 0.45, 1000.0
+
+
+def box():
+    return M.read_tetgen(os.path.join(M.GOLDEN, "model_box/box.1"))
+
+
+def unit_tet():
+    rest = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float32)
+    return rest, np.array([[0, 1, 2, 3]], np.int32)
+
+
+def rng_F(rng, n, spread=0.3, allow_inverted=False):
+    out = []
+    while len(out) < n:
+        F = np.eye(3) + spread * rng.standard_normal((3, 3))
+        if allow_inverted or np.linalg.det(F) > 0.2:
+            out.append(F)
+    return out
+
+
+def P_of_F(F, mu, lam, material=O.MAT_NEOHOOKEAN):
+    """First Piola stress from the oracle's per-tet forces on the unit right tet
+    (Dm = I, vol = 1/6, AN = -I/6): columns 1..3 of P AN = -P/6."""
+    rest, tets = unit_tet()
+    pre = O.precompute(rest, tets, np.zeros(4, np.int32))
+    x = np.zeros((4, 3))
+    x[1:] = np.asarray(F).T  # x_i = F X_i
+    f, _ = O.tet_forces(x, tets, pre, mu, lam, material, O.CLAMP_NONE, 64)
+    f = f.reshape(4, 3)
+    return -6.0 * f[1:].T
+
+
+# ---------------------------------------------------------------------------------------------
+def test_lame():
+    mu, lam = O.lame(E_, NU)
+    # IsotropicNeohookeanMaterial.cpp:15-16: mu = E/(2(1+nu)) = 34,482.76; lambda = 2mu/(1-2nu)
+    assert mu == pytest.approx(34482.758620689655, rel=1e-7)
+    assert lam == pytest.approx(689655.1724137931, rel=1e-6)  # FP32 (1 - 2 nu)
+    assert lam != pytest.approx(E_ * NU / ((1 + NU) * (1 - 2 * NU)), rel=1e-3)  # the quirk is kept
+
+
+def test_precompute_closed_forms_bar():
+    rest, tets = M.bar(3, 2, 4, 0.05)
+    mask = M.clamp_x0(rest)
+    pre = O.precompute(rest, tets, mask, RHO, (0.0, -9.8, 0.0), 1e8)
+    assert np.allclose(pre["vol"], 0.05 ** 3 / 6, rtol=1e-6)
+    m_orig = pre["gforce"][:, 1] / -9.8
+    assert m_orig.sum() == pytest.approx(RHO * 3 * 2 * 4 * 0.05 ** 3, rel=1e-5)
+    assert np.all(pre["mass"][mask == 1] == np.float32(1e8))
+    assert np.allclose(pre["mass"][mask == 0], m_orig[mask == 0], rtol=1e-6)
+    assert np.allclose(pre["gforce"][:, [0, 2]], 0)
+    r = rest.astype(np.float64)
+    Dm = np.stack([r[tets[:, k]] - r[tets[:, 0]] for k in (1, 2, 3)], -1)
+    Dmi = pre["dminv"].reshape(-1, 3, 3).astype(np.float64)
+    assert np.allclose(Dmi @ Dm, np.eye(3), atol=1e-5)
+    # AN = -vol Dm^-T for positively oriented tets
+    AN = pre["an"].reshape(-1, 3, 3)
+    assert np.allclose(AN, -pre["vol"][:, None, None] * np.transpose(Dmi, (0, 2, 1)), rtol=1e-4, atol=1e-9)
+
+
+@pytest.mark.parametrize("name", ["model_box/box.1", "model_bar/bar.1", "model_tet2/tet.2"])
+def test_precompute_model_meshes(name):
+    rest, tets = M.read_tetgen(os.path.join(M.GOLDEN, name))
+    pre = O.precompute(rest, tets, np.zeros(len(rest), np.int32))
+    r = rest.astype(np.float64)
+    Dm = np.stack([r[tets[:, k]] - r[tets[:, 0]] for k in (1, 2, 3)], -1)
+    vol = np.abs(np.linalg.det(Dm)) / 6
+    assert np.allclose(pre["vol"], vol, rtol=1e-5)
+    hull_like = vol.sum()
+    assert pre["mass"].sum() == pytest.approx(RHO * hull_like, rel=1e-5)
+    AN = pre["an"].reshape(-1, 3, 3)
+    assert np.allclose(AN, -vol[:, None, None] * np.transpose(np.linalg.inv(Dm), (0, 2, 1)), rtol=1e-3, atol=1e-7)
+
+
+def test_forces_zero_at_rest_and_uniform_scaling():
+    mu, lam = O.lame(E_, NU)
+    rest, tets = M.bar(2, 2, 3, 0.05)
+    pre = O.precompute(rest, tets, np.zeros(len(rest), np.int32))
+    f, e = O.tet_forces(rest.astype(np.float64), tets, pre, mu, lam, O.MAT_NEOHOOKEAN, O.CLAMP_NONE, 64)
+    scale = mu * 0.05 ** 2
+    assert np.abs(f).max() <= 1e-5 * scale and np.abs(e).max() <= 1e-9
+    for s in (0.9, 1.1, 1.5):
+        x = s * rest.astype(np.float64)
+        f, e = O.tet_forces(x, tets, pre, mu, lam, O.MAT_NEOHOOKEAN, O.CLAMP_NONE, 64)
+        p = mu * (s - 1 / s) + lam * np.log(s ** 3) / s  # P(sI) = p I
+        AN = pre["an"].reshape(-1, 3, 3).astype(np.float64)
+        expect = p * AN  # columns = forces on nodes 1..3
+        got = f.reshape(-1, 4, 3)[:, 1:, :].transpose(0, 2, 1)
+        assert np.allclose(got, expect, rtol=1e-5, atol=1e-9 * abs(p))
+        psi = mu / 2 * (3 * s * s - 3) - mu * np.log(s ** 3) + lam / 2 * np.log(s ** 3) ** 2
+        assert np.allclose(e, psi * pre["vol"], rtol=1e-5)
+
+
+@pytest.mark.parametrize("material", [O.MAT_NEOHOOKEAN, O.MAT_STRETCH])
+def test_forces_rotation_invariance_and_balance(material):
+    mu, lam = O.lame(E_, NU)
+    rest, tets = box()
+    pre = O.precompute(rest, tets, np.zeros(len(rest), np.int32))
+    rng = np.random.default_rng(0)
+    x = rest.astype(np.float64) + 0.05 * rng.standard_normal(rest.shape)
+    f, e = O.tet_forces(x, tets, pre, mu, lam, material, O.CLAMP_NONE, 64)
+    q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    if np.linalg.det(q) < 0:
+        q[:, 0] *= -1
+    fr, er = O.tet_forces(x @ q.T, tets, pre, mu, lam, material, O.CLAMP_NONE, 64)
+    assert np.allclose(er, e, rtol=1e-9)
+    assert np.allclose(fr.reshape(-1, 3), f.reshape(-1, 3) @ q.T, atol=1e-9 * np.abs(f).max())
+    f4 = f.reshape(-1, 4, 3)
+    assert np.abs(f4.sum(1)).max() <= 1e-9 * np.abs(f).max()
+    torque = np.cross(x[tets], f4).sum(1)
+    assert np.abs(torque).max() <= 1e-9 * np.abs(f).max()
+
+
+@pytest.mark.parametrize("material", [O.MAT_NEOHOOKEAN, O.MAT_STRETCH])
+def test_forces_are_minus_energy_gradient(material):
+    mu, lam = O.lame(E_, NU)
+    rest, tets = box()
+    pre = O.precompute(rest, tets, np.zeros(len(rest), np.int32))
+    rng = np.random.default_rng(1)
+    x = rest.astype(np.float64) + 0.04 * rng.standard_normal(rest.shape)
+
+    def energy(y):
+        return O.tet_forces(y, tets, pre, mu, lam, material, O.CLAMP_NONE, 64)[1].sum()
+
+    f, _ = O.tet_forces(x, tets, pre, mu, lam, material, O.CLAMP_NONE, 64)
+    fn = np.zeros_like(x)
+    np.add.at(fn, tets.reshape(-1), f.reshape(-1, 3))
+    h = 1e-6
+    fd = np.zeros_like(x)
+    for i in range(len(x)):
+        for d in range(3):
+            a, b = x.copy(), x.copy()
+            a[i, d] += h
+            b[i, d] -= h
+            fd[i, d] = -(energy(a) - energy(b)) / (2 * h)
+    assert np.abs(fn - fd).max() <= 1e-5 * np.abs(fn).max()
+
+
+def test_stretch_P_matches_numpy_polar():
+    mu, lam = O.lame(E_, NU)
+    rng = np.random.default_rng(2)
+    for F in rng_F(rng, 20):
+        U, S, Vt = np.linalg.svd(F)
+        Rm = U @ Vt
+        if np.linalg.det(Rm) < 0:
+            U[:, 2] *= -1
+            Rm = U @ Vt
+        J = np.linalg.det(F)
+        expect = 2 * mu * (F - Rm) + lam * (J - 1) * J * np.linalg.inv(F).T
+        # AN is FP32 mesh data (-1/6 rounds to 3e-8 relative)
+        assert np.allclose(P_of_F(F, mu, lam, O.MAT_STRETCH), expect, rtol=1e-7, atol=1e-7 * np.abs(expect).max())
+
+
+def test_svd_conventions():
+    rng = np.random.default_rng(3)
+    mats = rng_F(rng, 50, 0.6, allow_inverted=True) + [np.eye(3), 2 * np.eye(3), np.diag([1.0, 1.0, -1.0]),
+                                                        np.diag([3.0, 1.0, 1.0]), np.diag([1e-3, 1.0, 2.0])]
+    for F in mats:
+        for prec, tol in ((64, 1e-12), (32, 2e-6)):
+            U, S, V = O.svd3(F, prec)
+            U, S, V = U.astype(np.float64), S.astype(np.float64), V.astype(np.float64)
+            assert np.allclose(U @ np.diag(S) @ V.T, F, atol=tol * 10 * max(1, np.abs(F).max()))
+            assert np.allclose(U.T @ U, np.eye(3), atol=tol * 10) and np.allclose(V.T @ V, np.eye(3), atol=tol * 10)
+            assert np.linalg.det(U) > 0 and np.linalg.det(V) > 0
+            assert S[0] >= S[1] - tol and S[1] >= abs(S[2]) - tol
+            assert np.allclose(np.abs(S), np.linalg.svd(F, compute_uv=False), atol=tol * 10)
+            assert (S[2] < 0) == (np.linalg.det(F) < 0) or abs(S[2]) < 1e-9
+
+
+def closed_form_dpdf(F, mu, lam):
+    """Neo-Hookean dP_ij/dF_kl = mu d_ik d_jl + (mu - lam ln J) Fi_li Fi_jk + lam Fi_ji Fi_lk."""
+    Fi = np.linalg.inv(F)
+    lj = np.log(np.linalg.det(F))
+    I = np.eye(3)
+    T = (mu * np.einsum("ik,jl->ijkl", I, I) + (mu - lam * lj) * np.einsum("li,jk->ijkl", Fi, Fi)
+         + lam * np.einsum("ji,lk->ijkl", Fi, Fi))
+    return T.reshape(9, 9)
+
+
+def test_dpdf_matches_closed_form_and_fd():
+    mu, lam = O.lame(E_, NU)
+    rng = np.random.default_rng(4)
+    for F in rng_F(rng, 10):
+        D = O.dpdf(F, mu, lam, O.MAT_NEOHOOKEAN, 64)
+        C = closed_form_dpdf(F, mu, lam)
+        assert np.allclose(D, C, rtol=1e-7, atol=1e-6 * np.abs(C).max())
+        h = 1e-6
+        fd = np.zeros((9, 9))
+        for c in range(9):
+            a, b = F.copy().reshape(9), F.copy().reshape(9)
+            a[c] += h
+            b[c] -= h
+            fd[:, c] = (P_of_F(a.reshape(3, 3), mu, lam) - P_of_F(b.reshape(3, 3), mu, lam)).reshape(9) / (2 * h)
+        assert np.allclose(D, fd, rtol=1e-5, atol=1e-5 * np.abs(C).max())
+
+
+@pytest.mark.parametrize("F", [np.eye(3), 1.2 * np.eye(3), np.diag([1.1, 1.1, 0.9]), np.diag([1.3, 0.9, 0.9])])
+def test_dpdf_degenerate_singular_values(F):
+    # repeated singular values take the reference's 1e-6 perturbation: O(1e-6) relative change
+    mu, lam = O.lame(E_, NU)
+    D = O.dpdf(F, mu, lam, O.MAT_NEOHOOKEAN, 64)
+    C = closed_form_dpdf(F, mu, lam)
+    assert np.allclose(D, C, atol=1e-5 * np.abs(C).max())
+
+
+@pytest.mark.parametrize("material", [O.MAT_NEOHOOKEAN, O.MAT_STRETCH])
+def test_hessian_diag_fd(material):
+    mu, lam = O.lame(E_, NU)
+    rest, tets = box()
+    pre = O.precompute(rest, tets, np.zeros(len(rest), np.int32))
+    rng = np.random.default_rng(5)
+    x = rest.astype(np.float64) + 0.04 * rng.standard_normal(rest.shape)
+    hd, th = O.hessian_diag(x, tets, pre, mu, lam, material, 64)
+
+    def nodef(y):
+        f, _ = O.tet_forces(y, tets, pre, mu, lam, material, O.CLAMP_NONE, 64)
+        out = np.zeros_like(y)
+        np.add.at(out, tets.reshape(-1), f.reshape(-1, 3))
+        return out
+
+    h = 1e-6
+    fd = np.zeros_like(x)
+    for i in range(len(x)):
+        for d in range(3):
+            a, b = x.copy(), x.copy()
+            a[i, d] += h
+            b[i, d] -= h
+            fd[i, d] = -(nodef(a)[i, d] - nodef(b)[i, d]) / (2 * h)
+    assert np.allclose(hd, fd, rtol=1e-5)
+    tot = np.zeros_like(x)
+    np.add.at(tot, tets.reshape(-1), th.reshape(-1, 3))
+    assert np.allclose(tot, hd, rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------------------------
+# descent loop
+def test_step_mass_only_closed_form():
+    """With E = 0 the incremental potential is 1/2 m |x - x~|^2 / h^2 - m g.x: minimiser
+    x* = x~ + h^2 g (REF_GPU) and x~ + 2 h^2 g for REF_CPU, whose gradient counts gravity twice.
+    Plain damped Jacobi (empty Chebyshev profile): the profile is tuned for rho = 0.999, and on
+    this rho = 0.9 problem its over-relaxation oscillates and trips the energy check."""
+    rest, tets = M.bar(2, 2, 2, 0.05)
+    mask = M.clamp_x0(rest)
+    h = 0.03
+    for sem, factor in ((O.REF_GPU, 1.0), (O.REF_CPU, 2.0)):
+        # REF_CPU's gradient (2x gravity) and energy (1x) disagree past x~ + h^2 g, so its energy
+        # check would roll back there: check the energy only at k = 0 for this closed form.
+        check = 8 if sem == O.REF_GPU else 10 ** 6
+        p = O.default_params(semantics=sem, relgrad_tol=1e-9, iterations=20000, profile=[], energy_check=check)
+        o = O.Solver(rest, tets, mask, E=1e-30, params=p, precision=64)
+        st = o.step()
+        x, v, _ = o.state()
+        free = mask == 0
+        expect = rest.astype(np.float64) + factor * h * h * np.array([0, -9.8, 0])
+        # gravity force is FP32 data (m g rounded): 6e-8 relative of the 9 mm displacement
+        assert np.allclose(x[free], expect[free], atol=2e-9)
+        assert np.array_equal(x[~free], rest[~free].astype(np.float64))
+        assert np.allclose(v[free], 0.99 * factor * h * np.array([0, -9.8, 0]), rtol=1e-7, atol=1e-9)
+        assert st.converged == 1
+
+
+def incremental_potential(rest, tets, mask, xt, h, mu, lam, grav):
+    """Independent numpy incremental potential (elastic neo-Hookean + inertia + gravity)."""
+    r = rest.astype(np.float64)
+    Dm = np.stack([r[tets[:, k]] - r[tets[:, 0]] for k in (1, 2, 3)], -1)
+    Dmi = np.linalg.inv(Dm)
+    vol = np.abs(np.linalg.det(Dm)) / 6
+    m = np.zeros(len(rest))
+    np.add.at(m, tets.reshape(-1), np.repeat(vol * RHO / 4, 4))
+    free = mask == 0
+
+    def E(xf):
+        x = r.copy()
+        x[free] = xf.reshape(-1, 3)
+        Ds = np.stack([x[tets[:, k]] - x[tets[:, 0]] for k in (1, 2, 3)], -1)
+        F = Ds @ Dmi
+        J = np.linalg.det(F)
+        if (J <= 0).any():
+            return 1e30
+        lj = np.log(J)
+        psi = mu / 2 * ((F ** 2).sum((1, 2)) - 3) - mu * lj + lam / 2 * lj ** 2
+        d = x - xt
+        kin = 0.5 * (m[:, None] * d * d).sum(1) / h / h - m * (x @ np.asarray(grav))
+        return (psi * vol).sum() + kin[free].sum()
+
+    return E, free
+
+
+def test_step_converges_to_independent_minimiser():
+    mu, lam = O.lame(E_, NU)
+    rest, tets = box()
+    mask = (rest[:, 0] < -0.4).astype(np.int32)
+    assert 0 < mask.sum() < len(rest)
+    rng = np.random.default_rng(6)
+    x0 = rest + (0.02 * rng.standard_normal(rest.shape)).astype(np.float32) * (mask == 0)[:, None]
+    h = 0.03
+    # below ~1e-8 the FP64 energy differences of the line search reach rounding noise
+    p = O.default_params(relgrad_tol=1e-7, iterations=400000, profile=[], alpha0=0.5)
+    o = O.Solver(rest, tets, mask, params=p, precision=64)
+    o.set_state(x=x0.astype(np.float64))
+    st = o.step()
+    assert st.converged == 1
+    x_or = o.state()[0]
+    E, free = incremental_potential(rest, tets, mask, x0.astype(np.float64), h, mu, lam, (0, -9.8, 0))
+    res = minimize(E, x0[free].astype(np.float64).reshape(-1), method="BFGS", options={"gtol": 1e-10, "maxiter": 5000})
+    x_sp = res.x.reshape(-1, 3)
+    assert np.abs(x_or[free] - x_sp).max() <= 1e-6
+    assert E(x_or[free].reshape(-1)) <= res.fun + 1e-9
+
+
+def test_step_accepted_energies_never_increase_and_omega_schedule():
+    rest, tets = M.bar(4, 4, 16, 0.05)
+    mask = M.clamp_x0(rest)
+    x0 = M.twisted_state(rest, 4, 4, 16, 0.05)
+    p = O.default_params(trace_cap=200)
+    o = O.Solver(rest, tets, mask, params=p, precision=64)
+    o.set_state(x=x0.astype(np.float64))
+    st, tr = o.step(trace=True)
+    acc = [t.energy for t in tr if t.checked and not t.rollback]
+    assert all(b <= a for a, b in zip(acc, acc[1:]))
+    om = {t.k: t.omega for t in tr if not t.rollback}
+    # profile (1, .96), (7, .991), (10, .999): omega = 1 at k = 0, k_i - 1; 2/(2 - rho^2) at k_i
+    assert om[0] == 1.0 and om[6] == 1.0 and om[9] == 1.0
+    for k, rho in ((1, 0.96), (7, 0.991), (10, 0.999)):
+        assert om[k] == pytest.approx(2 / (2 - rho * rho), rel=1e-6)
+    # the recurrence omega = 4/(4 - rho^2 omega) converges to the optimal 2/(1 + sqrt(1 - rho^2))
+    star = 2 / (1 + np.sqrt(1 - 0.999 ** 2))
+    d = [abs(om[k] - star) for k in range(11, 96)]
+    assert all(b <= a + 1e-7 for a, b in zip(d, d[1:])) and d[-1] < 2e-3 * star
+
+
+def find_rollback_case():
+    rest, tets = M.bar(4, 4, 16, 0.05)
+    mask = M.clamp_x0(rest)
+    x0 = M.twisted_state(rest, 4, 4, 16, 0.05, 30.0)
+    return rest, tets, mask, x0
+
+
+def test_step_rollback_rules():
+    """Reference rule (CUDAInterface.cu:946-958): on E_k > E_old at a check, alpha *= 0.7,
+    iterations -= k - 8, x <- x_old and the same k is redone."""
+    rest, tets, mask, x0 = find_rollback_case()
+    for a0 in (0.8, 0.6, 0.4):
+        p = O.default_params(alpha0=a0, trace_cap=400)
+        o = O.Solver(rest, tets, mask, params=p, precision=64)
+        o.set_state(x=x0.astype(np.float64))
+        st, tr = o.step(trace=True)
+        if st.rollbacks:
+            break
+    assert st.rollbacks > 0, "no rollback case found"
+    budget = 96
+    for i, t in enumerate(tr):
+        if t.rollback:
+            assert t.checked and t.k % 8 == 0 and t.k > 0
+            assert tr[i + 1].k == t.k  # same k redone
+            budget -= t.k - 8
+    assert st.iterations == min(budget, st.iterations) and st.iterations <= budget
+    assert st.alpha == pytest.approx(a0 * 0.7 ** st.rollbacks, rel=1e-5)
+    assert st.passes == st.iterations + st.rollbacks
+
+
+def test_relgrad_mode_restarts_chebyshev_and_rejects_nan():
+    rest, tets = M.bar(4, 4, 16, 0.05)
+    mask = M.clamp_x0(rest)
+    x0 = M.twisted_state(rest, 4, 4, 16, 0.05)
+    p = O.default_params(relgrad_tol=1e-4, iterations=5000, trace_cap=6000)
+    o = O.Solver(rest, tets, mask, params=p, precision=64)
+    o.set_state(x=x0.astype(np.float64))
+    for _ in range(3):
+        st, tr = o.step(trace=True)
+        assert st.converged == 1 and st.g_norm <= 1e-4 * st.g0_norm
+        for i, t in enumerate(tr[:-1]):
+            if t.rollback:
+                assert tr[i + 1].omega == 1.0  # profile restarted
+    # FP32 state cannot reach 1e-4 (position rounding): documented precision limit
+    o32 = O.Solver(rest, tets, mask, params=O.default_params(relgrad_tol=1e-4, iterations=3000), precision=32)
+    o32.set_state(x=x0.astype(np.float64))
+    assert o32.step().converged == 0
