@@ -72,7 +72,22 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--max-iterations", type=int, default=40000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (read + write) for counter calibration")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        n = args.calibrate_copy * (1 << 20) // 4
+        a = torch.empty(n, device="cuda", dtype=torch.float32)
+        b = torch.empty_like(a)
+        a.fill_(1.0)  # elementwise fill kernel: writes 4n bytes
+        b.fill_(2.0)
+        c = a + b  # elementwise add kernel: reads 8n bytes, writes 4n bytes
+        torch.cuda.synchronize()
+        print(json.dumps({"fill_bytes_written": 4 * n, "add_bytes_read": 8 * n, "add_bytes_written": 4 * n,
+                          "check": float(c[0])}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the descent solver has no multi-GPU path in this build "
               "(see DESIGN.md for how it would shard).", file=sys.stderr)
