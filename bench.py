@@ -28,11 +28,14 @@ WORKLOADS = {
 }
 
 
-def algorithmic_bytes(nt, nn):
-    """Compulsory HBM bytes of one descent iteration: 56 B per tet (4 int32 indices, 9 FP32
-    Dm^-1 entries, 1 FP32 volume) and 68 B per node (read x, x~, x_last, Hessian diagonal + m/h^2,
-    mass, mask; write x_next), FP32 / int32."""
-    return {"k_tet": 56.0 * nt, "k_node": 68.0 * nn}
+def algorithmic_bytes(nt, nn, rsz):
+    """Compulsory HBM bytes per launch of the two kernels of a descent pass (rsz = 4 or 8 bytes
+    per position component).
+    k_tile: per tet its 4 node indices (int32) and rest data (9 FP32 Dm^-1 entries + FP32 volume)
+            = 56 B; per node the position read and the force written (2 x 3 rsz).
+    k_node: per node the reads of x, x~, x_last and the force (4 x 3 rsz), Hessian diagonal
+            (3 x FP32), mass (FP32) and mask (int32), and the write of x_next (3 rsz)."""
+    return {"k_tile": 56.0 * nt + 6.0 * rsz * nn, "k_node": (15.0 * rsz + 20.0) * nn}
 
 
 def cpu_baseline(rest, tets, mask, material, iters_per_step, grav, seconds=12.0):
@@ -89,8 +92,8 @@ def main():
                           "check": float(c[0])}))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: the descent solver has no multi-GPU path in this build "
-              "(see DESIGN.md for how it would shard).", file=sys.stderr)
+        print("bench.py runs on one GPU only: the descent solver is a single-GPU path; "
+              "multi-GPU runs are not supported.", file=sys.stderr)
         sys.exit(2)
 
     import phybs_amd as P
@@ -122,8 +125,8 @@ def main():
     iters = sum(st.iterations for st in stats)
     passes = sum(st.passes for st in stats)
     nt, nn = len(tets), len(rest)
-    alg = algorithmic_bytes(nt, nn)
-    dom = max(("k_tet", "k_node"), key=lambda k: kt[k][0] * kt[k][1])
+    alg = algorithmic_bytes(nt, nn, 8 if relg else 4)
+    dom = max(("k_tile", "k_node"), key=lambda k: kt[k][0] * kt[k][1])
     avg_ms = kt[dom][0]
     achieved = alg[dom] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     steps_per_s = args.steps / elapsed

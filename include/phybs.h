@@ -68,7 +68,9 @@ typedef struct pbs_params {
   int iterations;       /* descent iterations per step, 96 (fvm.cpp:187) */
   double relgrad_tol;   /* 0: fixed iterations; > 0: iterate until |g_k| <= tol |g_0| (free DOFs, FP64) */
   int max_iterations;   /* iteration cap in rel-grad mode */
-  int block_size;       /* threads per workgroup, 0 = tuned default */
+  int block_size;       /* threads per workgroup of the per-tet / per-node utility kernels, 0 = 256 */
+  int tile_nodes;       /* nodes owned per workgroup of the fused descent kernel (64..1024), 0 = by mesh size */
+  int chunk;            /* tets staged in LDS per round of the fused kernel (128, 256, 512), 0 = largest that fits */
 } pbs_params;
 
 typedef struct pbs_step_stats {

@@ -356,8 +356,9 @@ static void FN(forces_at)(orc_solver *s, FN(orc_state) *st, const R *y) {
 #pragma omp parallel for schedule(static)
   for (int n = 0; n < s->nn; ++n) {
     R acc[3] = {0, 0, 0};
-    for (int e = s->inc_off[n]; e < s->inc_off[n + 1]; ++e) {
-      int slot = s->inc[e]; /* 12*tet + 3*local */
+    const int e0 = s->inc_off[n], e1 = s->inc_off[n + 1];
+    for (int q = 0; q < e1 - e0; ++q) {
+      int slot = s->inc[s->p.gather_order ? e1 - 1 - q : e0 + q]; /* 12*tet + 3*local */
       for (int d = 0; d < 3; ++d) acc[d] += st->tetf[slot + d];
     }
     for (int d = 0; d < 3; ++d) st->nodef[3 * n + d] = acc[d];
@@ -369,8 +370,9 @@ static void FN(hd_at)(orc_solver *s, FN(orc_state) *st, const R *y) {
 #pragma omp parallel for schedule(static)
   for (int n = 0; n < s->nn; ++n) {
     R acc[3] = {0, 0, 0};
-    for (int e = s->inc_off[n]; e < s->inc_off[n + 1]; ++e) {
-      int slot = s->inc[e];
+    const int e0 = s->inc_off[n], e1 = s->inc_off[n + 1];
+    for (int q = 0; q < e1 - e0; ++q) {
+      int slot = s->inc[s->p.gather_order ? e1 - 1 - q : e0 + q];
       for (int d = 0; d < 3; ++d) acc[d] += st->tethd[slot + d];
     }
     for (int d = 0; d < 3; ++d) st->hd[3 * n + d] = acc[d];

@@ -50,6 +50,10 @@ typedef struct orc_params {
   int iterations;       /* per step (fvm.cpp:187 passes 96) */
   double relgrad_tol;   /* 0: fixed-iteration mode; >0: stop when |g_k|/|g_0| <= tol */
   int trace_cap;        /* number of per-pass trace records kept (0 = none) */
+  int gather_order;     /* per-node force / Hessian-diagonal sum: 0 = ascending tet id (the CPU
+                           path's tet loop order), 1 = descending.  The GPU path sums with float
+                           atomics (CUDAInterface.cu:526-541), i.e. in no fixed order; order 1
+                           measures how far FP32 results move with the summation order alone. */
 } orc_params;
 
 typedef struct orc_stats {

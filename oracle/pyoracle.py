@@ -23,7 +23,7 @@ class Params(C.Structure):
         ("energy_check", C.c_int), ("damping", C.c_float), ("init_momentum", C.c_float),
         ("profile_n", C.c_int), ("profile_k", C.c_int * 8), ("profile_rho", C.c_float * 8),
         ("semantics", C.c_int), ("iterations", C.c_int), ("relgrad_tol", C.c_double),
-        ("trace_cap", C.c_int),
+        ("trace_cap", C.c_int), ("gather_order", C.c_int),
     ]
 
 

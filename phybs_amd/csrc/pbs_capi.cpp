@@ -3,15 +3,20 @@
 // (CUDAInterface.cu:641-1007).  Everything runs on the handle's stream; the only host round
 // trip inside a step is one read of the 128-byte device control block after the planned passes
 // (the reference reads an energy every 8 iterations, CUDAInterface.cu:942-944).
+//
+// Inside the solver nodes and tets are renumbered (Morton order, pbs_tiles.cpp); every array that
+// crosses the ABI is in the caller's order and is permuted on the device or the host.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "pbs_host.h"
 #include "pbs_internal.h"
+#include "pbs_tiles.h"
 #include "phybs.h"
 
 namespace pbs {
@@ -31,8 +36,8 @@ using namespace pbs;
   } while (0)
 
 namespace {
-enum KernelId { K_TET = 0, K_NODE, K_CONTROL, K_INIT, K_FINAL, K_COUNT };
-const char *kKernelNames[K_COUNT] = {"k_tet", "k_node", "k_control", "k_step_init", "k_step_final"};
+enum KernelId { K_TILE = 0, K_NODE, K_CONTROL, K_INIT, K_FINAL, K_COUNT };
+const char *kKernelNames[K_COUNT] = {"k_tile", "k_node", "k_control", "k_step_init", "k_step_final"};
 }  // namespace
 
 struct pbs_solver {
@@ -44,6 +49,9 @@ struct pbs_solver {
   int device = 0;
   DevMesh<float> m32{};
   DevMesh<double> m64{};
+  TileDev td{};
+  TilePlan plan;
+  size_t lds_budget = 65536;
   std::vector<float> rest_h;
   float *stage = nullptr;     // 3N float AoS staging
   double *stage64 = nullptr;  // 3N double AoS staging
@@ -84,6 +92,9 @@ static int validate_params(const pbs_params *p) {
   if (p->relgrad_tol < 0 || (p->relgrad_tol > 0 && p->max_iterations <= 0)) return set_error(PBS_ERR_ARG, "bad rel-grad settings");
   if (p->block_size != 0 && p->block_size != 64 && p->block_size != 128 && p->block_size != 256)
     return set_error(PBS_ERR_ARG, "block_size must be 0, 64, 128 or 256");
+  if (p->tile_nodes != 0 && (p->tile_nodes < 16 || p->tile_nodes > 1024))
+    return set_error(PBS_ERR_ARG, "tile_nodes must be 0 or in [16, 1024]");
+  if (p->chunk != 0 && p->chunk != CHUNK_TETS) return set_error(PBS_ERR_ARG, "chunk must be 0 or 256");
   return PBS_OK;
 }
 
@@ -157,12 +168,12 @@ static void timed_collect(pbs_solver *s) {
 // ---- state helpers ----
 template <typename R> static int upload_planes(pbs_solver *s, const float *aos, R *planes) {
   HIPCHK(hipMemcpyAsync(s->stage, aos, sizeof(float) * 3 * (size_t)s->nn, hipMemcpyHostToDevice, s->st));
-  Launch<R>::aos_to_planes(s->stage, planes, s->nn, s->st);
+  Launch<R>::aos_to_planes(s->stage, planes, mesh<R>(s).perm, s->nn, s->st);
   HIPCHK(hipGetLastError());
   return PBS_OK;
 }
 template <typename R> static int download_planes(pbs_solver *s, const R *planes, float *aos) {
-  Launch<R>::planes_to_aos(planes, s->stage, s->nn, s->st);
+  Launch<R>::planes_to_aos(planes, s->stage, mesh<R>(s).perm, s->nn, s->st);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(aos, s->stage, sizeof(float) * 3 * (size_t)s->nn, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
@@ -189,6 +200,7 @@ template <typename R> static int alloc_impl(pbs_solver *s) {
   rc |= dalloc(s, &m.vol, T);
   rc |= dalloc(s, &m.inc_off, N + 1);
   rc |= dalloc(s, &m.inc, 4 * T);
+  rc |= dalloc(s, &m.perm, N);
   rc |= dalloc(s, &m.mask, N);
   rc |= dalloc(s, &m.mass, N);
   rc |= dalloc(s, &m.gforce, 3 * N);
@@ -205,6 +217,34 @@ template <typename R> static int alloc_impl(pbs_solver *s) {
   rc |= dalloc(s, &m.etet, T / 64 + 2);
   rc |= dalloc(s, &m.enode, N / 64 + 2);
   rc |= dalloc(s, &m.gpart, N / 64 + 2);
+  // tile pass
+  const TilePlan &P = s->plan;
+  const size_t H = std::max<long>(P.halo_total, 1);
+  TileInfo *tiles;
+  int *halo, *bnd_off, *bnd;
+  uint4 *trec;
+  uint32_t *coff;
+  unsigned short *cinc;
+  rc |= dalloc(s, &tiles, (size_t)P.ntiles);
+  rc |= dalloc(s, &halo, H);
+  rc |= dalloc(s, &trec, 3 * T);
+  rc |= dalloc(s, &coff, P.chunk_off.size());
+  rc |= dalloc(s, &cinc, P.chunk_inc.size() + 8 * CHUNK_WORDS_MAX);  // a prefetch may read one block past the end
+  rc |= dalloc(s, &bnd_off, N + 1);
+  rc |= dalloc(s, &bnd, std::max<size_t>(P.bnd.size(), 1));
+  rc |= dalloc(s, &s->td.tpart, (size_t)P.ntiles);
+  rc |= dalloc(s, &s->td.npart, 2 * (N / 64 + 2));
+  rc |= dalloc(s, &m.fown, 3 * N);
+  rc |= dalloc(s, &m.hpart, 3 * H);
+  rc |= dalloc(s, &m.fownh, 3 * N);
+  rc |= dalloc(s, &m.hparth, 3 * H);
+  s->td.tiles = tiles;
+  s->td.halo = halo;
+  s->td.tet_rec = trec;
+  s->td.chunk_off = coff;
+  s->td.chunk_inc = cinc;
+  s->td.bnd_off = bnd_off;
+  s->td.bnd = bnd;
   rc |= dalloc(s, &m.ctrl, 1);
   rc |= dalloc(s, &m.flag, 1);
   rc |= dalloc(s, &m.tete, T);
@@ -213,21 +253,33 @@ template <typename R> static int alloc_impl(pbs_solver *s) {
   return rc ? PBS_ERR_HIP : PBS_OK;
 }
 
-template <typename R> static int setup_impl(pbs_solver *s, const int *tets, const int *mask, const std::vector<int> &off,
-                                            const std::vector<int> &slots) {
+template <typename R> static int setup_impl(pbs_solver *s, const int *mask) {
   DevMesh<R> &m = mesh<R>(s);
+  const TilePlan &P = s->plan;
   const size_t N = s->nn;
   std::vector<float> planes(3 * N);
   for (size_t i = 0; i < N; ++i)
-    for (int d = 0; d < 3; ++d) planes[d * N + i] = s->rest_h[3 * i + d];
-  if (s->nt > 0) HIPCHK(hipMemcpyAsync(m.tets, tets, sizeof(int) * 4 * (size_t)s->nt, hipMemcpyHostToDevice, s->st));
+    for (int d = 0; d < 3; ++d) planes[d * N + i] = s->rest_h[3 * (size_t)P.perm[i] + d];
+  if (s->nt > 0) HIPCHK(hipMemcpyAsync(m.tets, P.tets_new.data(), sizeof(int) * 4 * (size_t)s->nt, hipMemcpyHostToDevice, s->st));
   HIPCHK(hipMemcpyAsync(m.rest, planes.data(), sizeof(float) * 3 * N, hipMemcpyHostToDevice, s->st));
   std::vector<int> mk(N, 0);
   if (mask)
-    for (size_t i = 0; i < N; ++i) mk[i] = mask[i] ? 1 : 0;
+    for (size_t i = 0; i < N; ++i) mk[i] = mask[P.perm[i]] ? 1 : 0;
   HIPCHK(hipMemcpyAsync(m.mask, mk.data(), sizeof(int) * N, hipMemcpyHostToDevice, s->st));
-  HIPCHK(hipMemcpyAsync(m.inc_off, off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, s->st));
-  if (s->nt > 0) HIPCHK(hipMemcpyAsync(m.inc, slots.data(), sizeof(int) * 4 * (size_t)s->nt, hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemcpyAsync(m.perm, P.perm.data(), sizeof(int) * N, hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemcpyAsync(m.inc_off, P.inc_off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, s->st));
+  if (s->nt > 0) HIPCHK(hipMemcpyAsync(m.inc, P.inc.data(), sizeof(int) * 4 * (size_t)s->nt, hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemcpyAsync((void *)s->td.tiles, P.tiles.data(), sizeof(TileInfo) * P.tiles.size(), hipMemcpyHostToDevice, s->st));
+  if (!P.halo.empty())
+    HIPCHK(hipMemcpyAsync((void *)s->td.halo, P.halo.data(), sizeof(int) * P.halo.size(), hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemcpyAsync((void *)s->td.chunk_off, P.chunk_off.data(), sizeof(uint32_t) * P.chunk_off.size(), hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemsetAsync((void *)s->td.chunk_inc, 0, sizeof(uint16_t) * (P.chunk_inc.size() + 8 * CHUNK_WORDS_MAX), s->st));
+  if (!P.chunk_inc.empty())
+    HIPCHK(hipMemcpyAsync((void *)s->td.chunk_inc, P.chunk_inc.data(), sizeof(uint16_t) * P.chunk_inc.size(), hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemcpyAsync((void *)s->td.bnd_off, P.bnd_off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, s->st));
+  if (!P.bnd.empty())
+    HIPCHK(hipMemcpyAsync((void *)s->td.bnd, P.bnd.data(), sizeof(int) * P.bnd.size(), hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemsetAsync(m.fownh, 0, sizeof(float) * 3 * N, s->st));
   HIPCHK(hipMemsetAsync(m.flag, 0, sizeof(int), s->st));
   HIPCHK(hipMemsetAsync(m.hd, 0, sizeof(float) * 3 * N, s->st));
   HIPCHK(hipMemsetAsync(m.ext, 0, sizeof(float) * 3 * N, s->st));
@@ -235,6 +287,18 @@ template <typename R> static int setup_impl(pbs_solver *s, const int *tets, cons
   HIPCHK(hipGetLastError());
   Launch<R>::setup_nodes(m, s->dp, 256, s->st);
   HIPCHK(hipGetLastError());
+  if (s->nt > 0) {
+    unsigned short *tl = nullptr;
+    HIPCHK(hipMalloc((void **)&tl, sizeof(uint16_t) * P.tet_local.size()));
+    hipError_t e = hipMemcpyAsync(tl, P.tet_local.data(), sizeof(uint16_t) * P.tet_local.size(), hipMemcpyHostToDevice, s->st);
+    if (e == hipSuccess) {
+      tet_records(m.dmi, m.vol, s->nt, tl, (uint4 *)s->td.tet_rec, s->st);
+      e = hipGetLastError();
+    }
+    (void)hipStreamSynchronize(s->st);
+    (void)hipFree(tl);
+    if (e != hipSuccess) return set_error(PBS_ERR_HIP, std::string("tet records: ") + hipGetErrorString(e));
+  }
   int flag = 0;
   HIPCHK(hipMemcpyAsync(&flag, m.flag, sizeof(int), hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
@@ -309,10 +373,6 @@ extern "C" int pbs_create(int nn, const float *nodes, const float *rest, const i
   if (rc) return rc;
   for (long i = 0; i < 4L * nt; ++i)
     if (tets[i] < 0 || tets[i] >= nn) return set_error(PBS_ERR_MESH, "tet index out of range");
-  std::vector<int> off(nn + 1), ent(4 * (size_t)nt);
-  rc = pbs_incidence(nn, nt, tets, off.data(), ent.data());
-  if (rc) return rc;
-  for (auto &e : ent) e /= 3;  // 12*tet + 3*local -> 4*tet + local
   pbs_solver *s = new pbs_solver();
   s->nn = nn;
   s->nt = nt;
@@ -325,12 +385,52 @@ extern "C" int pbs_create(int nn, const float *nodes, const float *rest, const i
     return code;
   };
   if (hipGetDevice(&s->device) != hipSuccess) return fail(set_error(PBS_ERR_HIP, "no HIP device"));
+  {
+    int maxlds = 65536;
+    if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device) != hipSuccess) maxlds = 65536;
+    s->lds_budget = (size_t)maxlds;
+  }
+  // Tile size: 512 owned nodes, halved for small meshes (to keep >= 512 tiles) and while the tile
+  // pass would need more than 80 KiB of LDS (two workgroups per 160 KiB CU).
+  const size_t rsz = p->precision == PBS_FP64 ? 8 : 4;
+  auto lds_need = [&](int lcap) {
+    TileDev t{};
+    t.lcap = (std::max(lcap, 1) + 3) / 4 * 4;
+    return rsz == 8 ? Launch<double>::tile_lds(t) : Launch<float>::tile_lds(t);
+  };
+  int tn = p->tile_nodes;
+  const bool auto_tile = tn == 0;
+  if (auto_tile) {
+    tn = 512;
+    while (tn > 64 && nn / tn < 512) tn /= 2;
+  }
+  std::string msg;
+  for (;;) {
+    if (!build_tile_plan(nn, rest, nt, tets, tn, s->plan, msg)) {
+      if (auto_tile && tn > 16) {
+        tn /= 2;
+        continue;
+      }
+      return fail(set_error(PBS_ERR_MESH, msg));
+    }
+    const size_t need = lds_need(s->plan.lcap);
+    if (auto_tile && tn > 16 && need > std::min<size_t>(s->lds_budget, 81920)) {
+      tn /= 2;
+      continue;
+    }
+    if (need > s->lds_budget) return fail(set_error(PBS_ERR_ARG, "tile too large for the LDS of one workgroup: use smaller tile_nodes"));
+    break;
+  }
+  s->td.ntiles = s->plan.ntiles;
+  s->td.tile_nodes = tn;
+  s->td.lcap = (std::max(s->plan.lcap, 1) + 3) / 4 * 4;  // keeps the LDS regions after it 16-byte aligned
+  s->td.halo_total = s->plan.halo_total;
   if (hipStreamCreate(&s->st) != hipSuccess) return fail(set_error(PBS_ERR_HIP, "hipStreamCreate failed"));
   if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
     return fail(set_error(PBS_ERR_HIP, "hipEventCreate failed"));
   rc = p->precision == PBS_FP64 ? alloc_impl<double>(s) : alloc_impl<float>(s);
   if (rc) return fail(rc);
-  rc = p->precision == PBS_FP64 ? setup_impl<double>(s, tets, mask, off, ent) : setup_impl<float>(s, tets, mask, off, ent);
+  rc = p->precision == PBS_FP64 ? setup_impl<double>(s, mask) : setup_impl<float>(s, mask);
   if (rc) return fail(rc);
   if (nodes) {
     rc = p->precision == PBS_FP64 ? set_positions_impl<double>(s, nodes) : set_positions_impl<float>(s, nodes);
@@ -392,7 +492,7 @@ extern "C" int pbs_get_positions(pbs_solver *s, float *xyz) {
 }
 
 template <typename R> static int get_pos64_impl(pbs_solver *s, double *xyz) {
-  Launch<R>::planes_to_aos64(mesh<R>(s).xpos, s->stage64, s->nn, s->st);
+  Launch<R>::planes_to_aos64(mesh<R>(s).xpos, s->stage64, mesh<R>(s).perm, s->nn, s->st);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(xyz, s->stage64, sizeof(double) * 3 * (size_t)s->nn, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
@@ -431,15 +531,14 @@ extern "C" int pbs_get_velocities(pbs_solver *s, float *v, float *vl) {
 
 // ---- the step ----
 template <typename R> static void launch_pass(pbs_solver *s) {
-  DevMesh<R> &m = mesh<R>(s);
-  int i = timed_begin(s, K_TET);
-  Launch<R>::tet(m, s->dp, s->bs, -1, 0, s->st);
+  int i = timed_begin(s, K_TILE);
+  Launch<R>::tile_pass(mesh<R>(s), s->td, s->dp, s->st);
   timed_end(s, i);
   i = timed_begin(s, K_NODE);
-  Launch<R>::node(m, s->dp, s->bs, s->st);
+  Launch<R>::node_pass(mesh<R>(s), s->td, s->dp, s->st);
   timed_end(s, i);
   i = timed_begin(s, K_CONTROL);
-  Launch<R>::control(m, s->dp, s->ntb, s->nnb, s->st);
+  Launch<R>::control(mesh<R>(s), s->td, s->dp, s->st);
   timed_end(s, i);
 }
 
@@ -447,7 +546,7 @@ template <typename R> static int step_impl(pbs_solver *s, const float *ext, floa
   DevMesh<R> &m = mesh<R>(s);
   if (ext) {
     HIPCHK(hipMemcpyAsync(s->stage, ext, sizeof(float) * 3 * (size_t)s->nn, hipMemcpyHostToDevice, s->st));
-    aos_to_planes_f(s->stage, m.ext, s->nn, s->st);
+    aos_to_planes_f(s->stage, m.ext, m.perm, s->nn, s->st);
     s->dp.has_ext = 1;
   } else {
     s->dp.has_ext = 0;
@@ -478,7 +577,7 @@ template <typename R> static int step_impl(pbs_solver *s, const float *ext, floa
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(s->ev1, s->st));
   if (out) {
-    Launch<R>::planes_to_aos(m.xpos, s->stage, s->nn, s->st);
+    Launch<R>::planes_to_aos(m.xpos, s->stage, m.perm, s->nn, s->st);
     HIPCHK(hipMemcpyAsync(out, s->stage, sizeof(float) * 3 * (size_t)s->nn, hipMemcpyDeviceToHost, s->st));
   }
   HIPCHK(hipStreamSynchronize(s->st));
@@ -521,15 +620,15 @@ extern "C" int pbs_do_descent_opt(pbs_solver *s, float h, int iterations, const 
 }
 
 // ---- material calls ----
-template <typename R> static int download_records(pbs_solver *s, const R *dev, float *host, size_t n) {
-  if (sizeof(R) == sizeof(float)) {
-    HIPCHK(hipMemcpyAsync(host, dev, sizeof(float) * n, hipMemcpyDeviceToHost, s->st));
-    HIPCHK(hipStreamSynchronize(s->st));
-  } else {
-    std::vector<R> tmp(n);
-    HIPCHK(hipMemcpyAsync(tmp.data(), dev, sizeof(R) * n, hipMemcpyDeviceToHost, s->st));
-    HIPCHK(hipStreamSynchronize(s->st));
-    for (size_t i = 0; i < n; ++i) host[i] = (float)tmp[i];
+// Per-tet records (width values per tet, solver tet order on the device) -> caller tet order.
+template <typename T> static int download_records(pbs_solver *s, const T *dev, float *host, int width) {
+  const size_t n = (size_t)width * s->nt;
+  std::vector<T> tmp(n);
+  HIPCHK(hipMemcpyAsync(tmp.data(), dev, sizeof(T) * n, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  for (int t = 0; t < s->nt; ++t) {
+    const size_t o = (size_t)width * s->plan.tperm[t];
+    for (int k = 0; k < width; ++k) host[o + k] = (float)tmp[(size_t)width * t + k];
   }
   return PBS_OK;
 }
@@ -547,11 +646,11 @@ template <typename R> static int forces_impl(pbs_solver *s, const float *pos, fl
     if (rc) return rc;
   }
   if (tf && s->nt > 0) {
-    rc = download_records<R>(s, m.frec, tf, 12 * (size_t)s->nt);
+    rc = download_records<R>(s, m.frec, tf, 12);
     if (rc) return rc;
   }
   if (te && s->nt > 0) {
-    rc = download_records<R>(s, m.tete, te, (size_t)s->nt);
+    rc = download_records<R>(s, m.tete, te, 1);
     if (rc) return rc;
   }
   HIPCHK(hipStreamSynchronize(s->st));
@@ -572,12 +671,15 @@ template <typename R> static int hd_impl(pbs_solver *s, const float *pos, float 
   HIPCHK(hipGetLastError());
   if (nh) {
     Launch<R>::gather(m, s->dp, s->bs, 1, m.hd, s->st);
-    planes_to_aos_f(m.hd, s->stage, s->nn, s->st);
+    planes_to_aos_f(m.hd, s->stage, m.perm, s->nn, s->st);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(nh, s->stage, sizeof(float) * 3 * (size_t)s->nn, hipMemcpyDeviceToHost, s->st));
   }
-  if (th && s->nt > 0) HIPCHK(hipMemcpyAsync(th, m.hrec, sizeof(float) * 12 * (size_t)s->nt, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
+  if (th && s->nt > 0) {
+    rc = download_records<float>(s, m.hrec, th, 12);
+    if (rc) return rc;
+  }
   return PBS_OK;
 }
 
@@ -588,19 +690,29 @@ extern "C" int pbs_compute_hessian_diag(pbs_solver *s, const float *pos, float *
 
 template <typename R> static int rest_data_impl(pbs_solver *s, float *dminv, float *vol, float *mass, float *gforce) {
   DevMesh<R> &m = mesh<R>(s);
+  const TilePlan &P = s->plan;
   const size_t N = s->nn, T = s->nt;
   if (dminv && T > 0) {
     std::vector<float> planes(9 * T);
     HIPCHK(hipMemcpyAsync(planes.data(), m.dmi, sizeof(float) * 9 * T, hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
     for (size_t t = 0; t < T; ++t)
-      for (int e = 0; e < 9; ++e) dminv[9 * t + e] = planes[e * T + t];
+      for (int e = 0; e < 9; ++e) dminv[9 * (size_t)P.tperm[t] + e] = planes[e * T + t];
   }
-  if (vol && T > 0) HIPCHK(hipMemcpyAsync(vol, m.vol, sizeof(float) * T, hipMemcpyDeviceToHost, s->st));
-  if (mass) HIPCHK(hipMemcpyAsync(mass, m.mass, sizeof(float) * N, hipMemcpyDeviceToHost, s->st));
-  HIPCHK(hipStreamSynchronize(s->st));
+  if (vol && T > 0) {
+    std::vector<float> v(T);
+    HIPCHK(hipMemcpyAsync(v.data(), m.vol, sizeof(float) * T, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    for (size_t t = 0; t < T; ++t) vol[P.tperm[t]] = v[t];
+  }
+  if (mass) {
+    std::vector<float> v(N);
+    HIPCHK(hipMemcpyAsync(v.data(), m.mass, sizeof(float) * N, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    for (size_t i = 0; i < N; ++i) mass[P.perm[i]] = v[i];
+  }
   if (gforce) {
-    planes_to_aos_f(m.gforce, s->stage, s->nn, s->st);
+    planes_to_aos_f(m.gforce, s->stage, m.perm, s->nn, s->st);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(gforce, s->stage, sizeof(float) * 3 * N, hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "pbs_tiles.h"
+
 namespace pbs {
 
 // Kernel-side copy of the solver constants (passed by value to every kernel).
@@ -38,18 +40,34 @@ struct Ctrl {
   double E, E_old, g0, g;
 };
 
+// Tile partition on the device (see pbs_tiles.h for the host-side construction).
+struct TileDev {
+  int ntiles, tile_nodes, lcap, pad;
+  long halo_total;
+  const TileInfo *tiles;
+  const int *halo;
+  const uint4 *tet_rec;         // 3 x uint4 per tet (solver order): {local ids 0|1, local ids 2|3,
+                                // Dm^-1[0..1]}, {Dm^-1[2..5]}, {Dm^-1[6..8], volume}
+  const uint32_t *chunk_off;    // per chunk (+1): offset in 16-bit words into chunk_inc
+  const unsigned short *chunk_inc;
+  const int *bnd_off, *bnd;     // per node: halo partial slots of its force, in tile order
+  double *tpart;                // per tile: elastic energy of its tets
+  double *npart;                // per node block: node energy, |g|^2
+};
+
 template <typename R> struct Ring {
   R *b[4];  // each: 3 planes of nn values (x | y | z)
 };
 
 // Device buffers of one solver instance.
 template <typename R> struct DevMesh {
-  int4 *tets;       // T
+  int4 *tets;       // T (solver order, solver node ids)
   float *rest;      // 3 planes
   float *dmi;       // 9 planes of T (Dm^-1 row-major entry e at dmi[e*T + t])
   float *vol;       // T
   int *inc_off;     // N+1
-  int *inc;         // 4T, entry = 4*tet + local
+  int *inc;         // 4T, entry = 4*tet + local, per node in ascending caller tet id
+  int *perm;        // N: solver node -> caller node
   int *mask;        // N
   float *mass;      // N (fixed nodes: fixed_mass)
   float *gforce;    // 3 planes: mass (before the fixed override) * grav
@@ -67,6 +85,8 @@ template <typename R> struct DevMesh {
   Ctrl *ctrl;
   int *flag;        // setup error flag
   R *tete;          // T, optional per-tet energy output
+  R *fown, *hpart;  // tile pass outputs: owned-node partial forces (3 planes of N), halo partials (3 x H)
+  float *fownh, *hparth;  // the same for the Hessian diagonal
 };
 
 // ---- launch wrappers (pbs_kernels.hip) ----
@@ -74,18 +94,21 @@ template <typename R> struct Launch {
   static void setup_tets(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st);
   static void setup_nodes(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st);
   static void tet(const DevMesh<R> &m, const DevParams &p, int bs, int force_hd, int write_e, hipStream_t st);
-  static void node(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st);
-  static void control(const DevMesh<R> &m, const DevParams &p, int ntb, int nnb, hipStream_t st);
+  static size_t tile_lds(const TileDev &td);
+  static void tile_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
+  static void node_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
+  static void control(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
   static void step_init(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st);
   static void step_final(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st);
   static void standalone_ctrl(const DevMesh<R> &m, int cur, hipStream_t st);
   static void gather(const DevMesh<R> &m, const DevParams &p, int bs, int which, void *out_planes, hipStream_t st);
-  static void aos_to_planes(const float *aos, R *planes, int n, hipStream_t st);
-  static void planes_to_aos(const R *planes, float *aos, int n, hipStream_t st);
-  static void planes_to_aos64(const R *planes, double *aos, int n, hipStream_t st);
+  static void aos_to_planes(const float *aos, R *planes, const int *perm, int n, hipStream_t st);
+  static void planes_to_aos(const R *planes, float *aos, const int *perm, int n, hipStream_t st);
+  static void planes_to_aos64(const R *planes, double *aos, const int *perm, int n, hipStream_t st);
 };
-void aos_to_planes_f(const float *aos, float *planes, int n, hipStream_t st);
-void planes_to_aos_f(const float *planes, float *aos, int n, hipStream_t st);
+void aos_to_planes_f(const float *aos, float *planes, const int *perm, int n, hipStream_t st);
+void planes_to_aos_f(const float *planes, float *aos, const int *perm, int n, hipStream_t st);
+void tet_records(const float *dmi, const float *vol, int T, const unsigned short *tet_local, uint4 *out, hipStream_t st);
 void fill_f(float *p, float v, long n, hipStream_t st);
 
 }  // namespace pbs

@@ -1,15 +1,17 @@
 // pbs_kernels.hip -- HIP kernels of the descent solver for gfx950 (CDNA4, wave64).
 //
-// One descent pass = k_tet (per-tet force, energy, and on refresh passes the Hessian
-// diagonal) -> k_node (atomic-free gather over the tet-incidence CSR, Jacobi step, Chebyshev
-// update, energy and |g|^2 partials) -> k_control (FP64 reduction of the partials and the
-// line-search / Chebyshev / buffer-role decisions on the device).  This replaces the
-// reference's computeForceKd / updateNodesNextIter / updateChebyshev / thrust::reduce and the
-// host loop of CUDAInterface::doDescentOpt (CUDAInterface.cu:482-1007) with no host round trip.
+// One descent pass = k_tile (each tet once, per tile: force, energy, Hessian diagonal on refresh
+// passes, gathered per node in LDS) -> k_node (owned + halo partials, Jacobi step, Chebyshev,
+// energy and |g|^2 partials) -> k_control (FP64 reduction in a fixed order and the line-search /
+// Chebyshev / buffer-role decisions).  This replaces the reference's computeForceKd /
+// updateNodesNextIter / updateChebyshev / thrust::reduce and the host loop of
+// CUDAInterface::doDescentOpt (CUDAInterface.cu:482-1007) with no host round trip and no atomics.
+// k_tet + k_gather evaluate forces and the Hessian diagonal for the standalone material calls.
 //
-// Layout: node vectors are 3 planes (x | y | z) of nn values (SoA); per-tet rest data are 10
-// planes of nt floats (Dm^-1 entries, volume); per-tet force records are 12 consecutive values
-// (4 local nodes x xyz) so that a node's gather reads 12 contiguous bytes per incident tet.
+// Layout: nodes are renumbered in Morton order and tets sorted by their smallest node; node
+// vectors are 3 planes (x | y | z) of nn values (SoA); per-tet rest data are 10 planes (Dm^-1
+// entries, volume) plus one 48-byte record per tet for the tile pass; per-tet force records of
+// the standalone calls are 12 consecutive values.
 #include "pbs_device.h"
 #include "pbs_internal.h"
 
@@ -174,125 +176,11 @@ __global__ void __launch_bounds__(256) k_tet(DevParams p, const Ctrl *__restrict
 }
 
 // ---------------------------------------------------------------------------------------------
-// Per-node pass: gather + Jacobi-preconditioned step + Chebyshev (updateNodesNextIter and
-// updateChebyshev, CUDAInterface.cu:572-623).  g = (x - x~) m/h^2 - f - s (m grav + f_ext)
-// with s = 1 (REF_GPU) or 2 (REF_CPU, DescentOptimize.cpp:51-52); fixed nodes keep x.
+// Loop control of the reference (CUDAInterface.cu:917-1003) applied on the device after the
+// pass over iterate x_k: E = incremental potential at x_k (FP64), g2 = |g(x_k)|^2.
 // ---------------------------------------------------------------------------------------------
-template <typename R>
-__global__ void __launch_bounds__(256) k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring,
-                                              const int *__restrict__ inc_off, const int *__restrict__ inc,
-                                              const R *__restrict__ frec, const float *__restrict__ hrec,
-                                              float *__restrict__ hd, const float *__restrict__ mass,
-                                              const int *__restrict__ mask, const float *__restrict__ gforce,
-                                              const float *__restrict__ ext, const R *__restrict__ xt,
-                                              double *__restrict__ enode, double *__restrict__ gpart) {
-  __shared__ double lds[16];
-  if (ctrl->done) return;
-  const int hdnow = ctrl->hd_now, cheb = ctrl->cheb_now;
-  const R alpha = (R)ctrl->alpha, om = (R)ctrl->omega_next;
-  const R *__restrict__ x = ring.b[ctrl->cur];
-  const R *__restrict__ xl = ring.b[ctrl->last];
-  R *__restrict__ xn = ring.b[ctrl->nxt];
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  const int N = p.nn;
-  double en = 0.0, g2 = 0.0;
-  if (n < N) {
-    const int b = inc_off[n], end = inc_off[n + 1];
-    R f0 = 0, f1 = 0, f2 = 0;
-    for (int e = b; e < end; ++e) {
-      const R *r = frec + 3 * (size_t)inc[e];
-      f0 += r[0];
-      f1 += r[1];
-      f2 += r[2];
-    }
-    float h0, h1, h2;
-    if (hdnow) {
-      h0 = h1 = h2 = 0.0f;
-      for (int e = b; e < end; ++e) {
-        const float *r = hrec + 3 * (size_t)inc[e];
-        h0 += r[0];
-        h1 += r[1];
-        h2 += r[2];
-      }
-      hd[n] = h0;
-      hd[N + n] = h1;
-      hd[2 * (size_t)N + n] = h2;
-    } else {
-      h0 = hd[n];
-      h1 = hd[N + n];
-      h2 = hd[2 * (size_t)N + n];
-    }
-    const R fx[3] = {f0, f1, f2};
-    const float hh[3] = {h0, h1, h2};
-    if (mask[n]) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) xn[(size_t)d * N + n] = x[(size_t)d * N + n];
-    } else {
-      const R hstep = (R)p.h;
-      const R mh2 = (R)mass[n] / (hstep * hstep);
-      R e = (R)0;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const size_t i = (size_t)d * N + n;
-        const R xc = x[i];
-        const R dd = xc - xt[i];
-        R fe = (R)gforce[i];
-        if (p.has_ext) fe += (R)ext[i];
-        const R g = dd * mh2 - fx[d] - (R)p.ext_scale * fe;
-        e += (R)0.5 * dd * dd * mh2 - fe * xc;
-        g2 += (double)g * (double)g;
-        R nx = xc - alpha * g / ((R)hh[d] + mh2);
-        if (cheb) {
-          const R l = xl[i];
-          nx = om * (nx - l) + l;
-        }
-        xn[i] = nx;
-      }
-      en = (double)e;
-    }
-  }
-  const double se = block_sum(en, lds);
-  const double sg = block_sum(g2, lds);
-  if (threadIdx.x == 0) {
-    enode[blockIdx.x] = se;
-    gpart[blockIdx.x] = sg;
-  }
-}
-
-// ---------------------------------------------------------------------------------------------
-// Control: FP64 sums of the partials in a fixed order, then the reference's loop logic
-// (CUDAInterface.cu:917-1003) on the device.
-// ---------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_control(DevParams p, Ctrl *__restrict__ ctrl,
-                                                 const double *__restrict__ etet, int ntb,
-                                                 const double *__restrict__ enode,
-                                                 const double *__restrict__ gpart, int nnb) {
-  __shared__ double la[256], lb[256], lc[256];
-  if (ctrl->done) return;
-  const int tid = threadIdx.x;
-  const int check = ctrl->check_now;
-  double a = 0.0, b = 0.0, c = 0.0;
-  if (check)
-    for (int i = tid; i < ntb; i += 256) a += etet[i];
-  for (int i = tid; i < nnb; i += 256) {
-    if (check) b += enode[i];
-    c += gpart[i];
-  }
-  la[tid] = a;
-  lb[tid] = b;
-  lc[tid] = c;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (tid < s) {
-      la[tid] += la[tid + s];
-      lb[tid] += lb[tid + s];
-      lc[tid] += lc[tid + s];
-    }
-    __syncthreads();
-  }
-  if (tid != 0) return;
-  Ctrl C = *ctrl;
-  const double g = sqrt(lc[0]);
+__device__ void control_apply(const DevParams &p, Ctrl &C, double E, double g2) {
+  const double g = sqrt(g2);
   C.g = g;
   if (C.first) {
     if (C.g0 < 0.0) C.g0 = g;
@@ -302,13 +190,11 @@ __global__ void __launch_bounds__(256) k_control(DevParams p, Ctrl *__restrict__
     C.converged = 1;
     C.done = 1;
     C.passes++;
-    *ctrl = C;
     return;
   }
   int rb = 0;
   const bool robust = p.relgrad_tol > 0.0;
-  if (check) {
-    const double E = la[0] + lb[0];
+  if (C.check_now) {
     C.E = E;
     // Rel-grad mode (no reference counterpart) also rejects a non-finite energy.
     const bool worse = robust ? !(E <= C.E_old) : (E > C.E_old);
@@ -346,7 +232,296 @@ __global__ void __launch_bounds__(256) k_control(DevParams p, Ctrl *__restrict__
     C.omega_next = om;
     C.cheb_now = om != 1.0f;
   }
-  *ctrl = C;
+}
+
+// XCD-aware tile order: workgroups are dispatched round-robin over the 8 XCDs, so block i runs
+// on XCD i % 8.  Giving each XCD a contiguous range of tiles keeps neighbouring tiles (which
+// share halo nodes and halo tets) in the same 4 MB L2.
+__device__ __forceinline__ int xcd_tile(int bid, int nblk) {
+  const int per = (nblk + 7) >> 3;
+  const int xcd = bid & 7, k = bid >> 3;
+  const int full = nblk - 8 * (per - 1);  // XCDs that get `per` tiles (the rest get per - 1)
+  const int start = xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1);
+  return start + k;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Tile pass (per-tet half of computeForceKd, CUDAInterface.cu:495-568): one workgroup per tile
+// evaluates each of the tile's tets once -- force, energy on check passes, Hessian diagonal on
+// refresh passes -- and gathers the results per local node in LDS in a fixed order (no atomics).
+// Outputs: a partial sum per owned node (fown) and per halo node (hpart).
+//   LDS: positions of local nodes (4 R each), force sums (4 R), Hessian sums (4 float), one
+//   chunk of per-tet records (256 tets x 4 corners x 4 R), one chunk's incidence block (8 KiB).
+//   The 48-byte records and the incidence block of chunk c+1 are loaded while chunk c computes.
+// ---------------------------------------------------------------------------------------------
+template <typename R, int MAT>
+__global__ void __launch_bounds__(256)
+k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, R *__restrict__ fown,
+       R *__restrict__ hpart, float *__restrict__ fownh, float *__restrict__ hparth) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double lds_red[16];
+  if (ctrl->done) return;
+  const int hdnow = ctrl->hd_now, chk = ctrl->check_now;
+  const R *__restrict__ x = ring.b[ctrl->cur];
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const TileInfo ti = td.tiles[b];
+  const int lcap = td.lcap, L = ti.own_cnt + ti.halo_cnt, N = p.nn, tid = threadIdx.x;
+  const long H = td.halo_total;
+  R *sx = (R *)smem;
+  R *facc = sx + 4 * lcap;
+  float *hacc = (float *)(facc + 4 * lcap);
+  R *rec = (R *)(hacc + 4 * lcap);
+  unsigned short *sinc = (unsigned short *)(rec + CHUNK_TETS * 16);
+  for (int i = tid; i < L; i += 256) {
+    const int g = i < ti.own_cnt ? ti.own_start + i : td.halo[ti.halo_off + i - ti.own_cnt];
+    sx[4 * i] = x[g];
+    sx[4 * i + 1] = x[N + g];
+    sx[4 * i + 2] = x[2 * (size_t)N + g];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      facc[4 * i + d] = (R)0;
+      hacc[4 * i + d] = 0.0f;
+    }
+  }
+  uint4 nr[3], ni[2] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
+  auto prefetch = [&](int c) {
+    if (c < ti.nchunk) {
+      const int t = ti.tet_start + c * CHUNK_TETS + tid;
+      if (c * CHUNK_TETS + tid < ti.tet_cnt) {
+        const uint4 *r = td.tet_rec + 3 * (size_t)t;
+        nr[0] = r[0];
+        nr[1] = r[1];
+        nr[2] = r[2];
+      }
+      const uint32_t o0 = td.chunk_off[ti.chunk_first + c], o1 = td.chunk_off[ti.chunk_first + c + 1];
+      const uint4 *src = (const uint4 *)(td.chunk_inc + o0);
+      const int nv = (int)(o1 - o0) >> 3;
+      if (tid < nv) ni[0] = src[tid];
+      if (tid + 256 < nv) ni[1] = src[tid + 256];
+    }
+  };
+  prefetch(0);
+  __syncthreads();
+  const R mu = (R)p.mu, lam = (R)p.lam;
+  double e_el = 0.0;
+  for (int c = 0; c < ti.nchunk; ++c) {
+    ((uint4 *)sinc)[tid] = ni[0];
+    ((uint4 *)sinc)[tid + 256] = ni[1];
+    const uint4 r0 = nr[0], r1 = nr[1], r2 = nr[2];
+    const bool have = c * CHUNK_TETS + tid < ti.tet_cnt;
+    prefetch(c + 1);
+    for (int phase = 0; phase <= hdnow; ++phase) {
+      if (have) {
+        const int a0 = r0.x & 0xffff, a1 = r0.x >> 16, a2 = r0.y & 0xffff, a3 = r0.y >> 16;
+        R Dmi[9];
+        Dmi[0] = (R)__uint_as_float(r0.z);
+        Dmi[1] = (R)__uint_as_float(r0.w);
+        Dmi[2] = (R)__uint_as_float(r1.x);
+        Dmi[3] = (R)__uint_as_float(r1.y);
+        Dmi[4] = (R)__uint_as_float(r1.z);
+        Dmi[5] = (R)__uint_as_float(r1.w);
+        Dmi[6] = (R)__uint_as_float(r2.x);
+        Dmi[7] = (R)__uint_as_float(r2.y);
+        Dmi[8] = (R)__uint_as_float(r2.z);
+        const R V = (R)__uint_as_float(r2.w);
+        const R *p0 = sx + 4 * a0, *p1 = sx + 4 * a1, *p2 = sx + 4 * a2, *p3 = sx + 4 * a3;
+        R Ds[9];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const R x0 = p0[k];
+          Ds[3 * k + 0] = p1[k] - x0;
+          Ds[3 * k + 1] = p2[k] - x0;
+          Ds[3 * k + 2] = p3[k] - x0;
+        }
+        R F[9];
+        mul3(Ds, Dmi, F);
+        R f12[12], h12[12], en;
+        R *rr = rec + 16 * tid;
+        if (phase == 0) {
+          if (MAT == 0) neo_tet<R, false>(F, Dmi, V, mu, lam, f12, en, h12);
+          else stretch_tet<R, false>(F, Dmi, V, mu, lam, f12, en, h12);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            rr[4 * q] = f12[3 * q];
+            rr[4 * q + 1] = f12[3 * q + 1];
+            rr[4 * q + 2] = f12[3 * q + 2];
+          }
+          if (chk) {
+            if (p.clamp == 1) en = en > (R)0 ? en : (R)0;          // CUDAInterface.cu:549
+            else if (p.clamp == 2) en = en > (R)1e-6 ? en : (R)0;  // IsotropicNeohookeanMaterial.cpp:129
+            e_el += (double)en;
+          }
+        } else {
+          if (MAT == 0) neo_tet<R, true>(F, Dmi, V, mu, lam, f12, en, h12);
+          else stretch_tet<R, true>(F, Dmi, V, mu, lam, f12, en, h12);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            rr[4 * q] = (R)(float)h12[3 * q];
+            rr[4 * q + 1] = (R)(float)h12[3 * q + 1];
+            rr[4 * q + 2] = (R)(float)h12[3 * q + 2];
+          }
+        }
+      }
+      __syncthreads();
+      const int nn = sinc[0];
+      const unsigned short *nodes = sinc + 1, *offs = sinc + 1 + nn, *ents = sinc + 2 + 2 * nn;
+      for (int i = tid; i < nn; i += 256) {
+        const int node = nodes[i];
+        R a0s = (R)0, a1s = (R)0, a2s = (R)0;
+        for (int e = offs[i]; e < offs[i + 1]; ++e) {
+          const int v = ents[e];
+          const R *rr = rec + 16 * (v >> 2) + 4 * (v & 3);
+          a0s += rr[0];
+          a1s += rr[1];
+          a2s += rr[2];
+        }
+        if (phase == 0) {
+          facc[4 * node] += a0s;
+          facc[4 * node + 1] += a1s;
+          facc[4 * node + 2] += a2s;
+        } else {
+          hacc[4 * node] += (float)a0s;
+          hacc[4 * node + 1] += (float)a1s;
+          hacc[4 * node + 2] += (float)a2s;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = tid; i < L; i += 256) {
+    if (i < ti.own_cnt) {
+      const size_t g = (size_t)ti.own_start + i;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) fown[d * (size_t)N + g] = facc[4 * i + d];
+      if (hdnow)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) fownh[d * (size_t)N + g] = hacc[4 * i + d];
+    } else {
+      const size_t h = (size_t)ti.halo_off + (i - ti.own_cnt);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) hpart[d * H + h] = facc[4 * i + d];
+      if (hdnow)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) hparth[d * H + h] = hacc[4 * i + d];
+    }
+  }
+  if (chk) {
+    const double se = block_sum(e_el, lds_red);
+    if (tid == 0) td.tpart[b] = se;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Node pass (updateNodesNextIter + updateChebyshev, CUDAInterface.cu:572-623): force = owned
+// partial + the node's halo partials in tile order; Jacobi-preconditioned step with Chebyshev.
+// g = (x - x~) m/h^2 - f - s (m grav + f_ext), s = 1 (REF_GPU) or 2 (REF_CPU, DescentOptimize.cpp:51-52).
+// ---------------------------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, const R *__restrict__ fown,
+       const R *__restrict__ hpart, const float *__restrict__ fownh, const float *__restrict__ hparth,
+       float *__restrict__ hd, const float *__restrict__ mass, const int *__restrict__ mask,
+       const float *__restrict__ ext, const R *__restrict__ xt) {
+  __shared__ double lds_red[16];
+  if (ctrl->done) return;
+  const int hdnow = ctrl->hd_now, cheb = ctrl->cheb_now;
+  const R alpha = (R)ctrl->alpha, om = (R)ctrl->omega_next;
+  const R *__restrict__ x = ring.b[ctrl->cur];
+  const R *__restrict__ xl = ring.b[ctrl->last];
+  R *__restrict__ xn = ring.b[ctrl->nxt];
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = p.nn;
+  const long H = td.halo_total;
+  double en = 0.0, g2 = 0.0;
+  if (n < N) {
+    R f[3];
+    float hv[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) f[d] = fown[(size_t)d * N + n];
+    const int b0 = td.bnd_off[n], b1 = td.bnd_off[n + 1];
+    for (int e = b0; e < b1; ++e) {
+      const size_t slot = td.bnd[e];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) f[d] += hpart[d * H + slot];
+    }
+    if (hdnow) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) hv[d] = fownh[(size_t)d * N + n];
+      for (int e = b0; e < b1; ++e) {
+        const size_t slot = td.bnd[e];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) hv[d] += hparth[d * H + slot];
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) hd[(size_t)d * N + n] = hv[d];
+    } else {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) hv[d] = hd[(size_t)d * N + n];
+    }
+    if (mask[n]) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) xn[(size_t)d * N + n] = x[(size_t)d * N + n];
+    } else {
+      const float m = mass[n];
+      const R hstep = (R)p.h;
+      const R mh2 = (R)m / (hstep * hstep);
+      R e = (R)0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const size_t i = (size_t)d * N + n;
+        const R xc = x[i];
+        const R dd = xc - xt[i];
+        R fe = (R)(p.grav[d] * m);  // = the setup's gravity force (mass before the fixed override)
+        if (p.has_ext) fe += (R)ext[i];
+        const R gr = dd * mh2 - f[d] - (R)p.ext_scale * fe;
+        e += (R)0.5 * dd * dd * mh2 - fe * xc;
+        g2 += (double)gr * (double)gr;
+        R nx = xc - alpha * gr / ((R)hv[d] + mh2);
+        if (cheb) {
+          const R l = xl[i];
+          nx = om * (nx - l) + l;
+        }
+        xn[i] = nx;
+      }
+      en = (double)e;
+    }
+  }
+  const double se = block_sum(en, lds_red);
+  const double sg = block_sum(g2, lds_red);
+  if (threadIdx.x == 0) {
+    td.npart[2 * blockIdx.x] = se;
+    td.npart[2 * blockIdx.x + 1] = sg;
+  }
+}
+
+// Control step: FP64 sums of the tile and node-block partials in a fixed order, then the loop
+// control of the reference.
+__global__ void __launch_bounds__(256) k_control(DevParams p, Ctrl *__restrict__ ctrl, TileDev td, int nnb) {
+  __shared__ double la[256], lc[256];
+  if (ctrl->done) return;
+  const int tid = threadIdx.x, chk = ctrl->check_now;
+  double a = 0.0, c = 0.0;
+  if (chk)
+    for (int i = tid; i < td.ntiles; i += 256) a += td.tpart[i];
+  for (int i = tid; i < nnb; i += 256) {
+    a += td.npart[2 * i];
+    c += td.npart[2 * i + 1];
+  }
+  la[tid] = a;
+  lc[tid] = c;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) {
+      la[tid] += la[tid + st];
+      lc[tid] += lc[tid + st];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    Ctrl C = *ctrl;
+    control_apply(p, C, la[0], lc[0]);
+    *ctrl = C;
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -460,20 +635,38 @@ __global__ void k_gather(int N, const int *__restrict__ inc_off, const int *__re
   out[2 * (size_t)N + n] = c;
 }
 
+// AoS (caller's node order) <-> planes (solver's node order); perm maps solver -> caller index.
 template <typename A, typename B>
-__global__ void k_aos_to_planes(const A *__restrict__ aos, B *__restrict__ planes, int n) {
+__global__ void k_aos_to_planes(const A *__restrict__ aos, B *__restrict__ planes, const int *__restrict__ perm, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const size_t o = 3 * (size_t)(perm ? perm[i] : i);
 #pragma unroll
-  for (int d = 0; d < 3; ++d) planes[(size_t)d * n + i] = (B)aos[3 * (size_t)i + d];
+  for (int d = 0; d < 3; ++d) planes[(size_t)d * n + i] = (B)aos[o + d];
 }
 
 template <typename A, typename B>
-__global__ void k_planes_to_aos(const A *__restrict__ planes, B *__restrict__ aos, int n) {
+__global__ void k_planes_to_aos(const A *__restrict__ planes, B *__restrict__ aos, const int *__restrict__ perm, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const size_t o = 3 * (size_t)(perm ? perm[i] : i);
 #pragma unroll
-  for (int d = 0; d < 3; ++d) aos[3 * (size_t)i + d] = (B)planes[(size_t)d * n + i];
+  for (int d = 0; d < 3; ++d) aos[o + d] = (B)planes[(size_t)d * n + i];
+}
+
+// Per-tet 48-byte record: tile-local node ids, then the tet's Dm^-1 (row-major) and volume.
+__global__ void k_tet_records(const float *__restrict__ dmi, const float *__restrict__ vol, int T,
+                              const unsigned short *__restrict__ loc, uint4 *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  float r[10];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) r[k] = dmi[(size_t)k * T + t];
+  r[9] = vol[t];
+  const unsigned l0 = loc[4 * (size_t)t], l1 = loc[4 * (size_t)t + 1], l2 = loc[4 * (size_t)t + 2], l3 = loc[4 * (size_t)t + 3];
+  out[3 * (size_t)t + 0] = make_uint4(l0 | (l1 << 16), l2 | (l3 << 16), __float_as_uint(r[0]), __float_as_uint(r[1]));
+  out[3 * (size_t)t + 1] = make_uint4(__float_as_uint(r[2]), __float_as_uint(r[3]), __float_as_uint(r[4]), __float_as_uint(r[5]));
+  out[3 * (size_t)t + 2] = make_uint4(__float_as_uint(r[6]), __float_as_uint(r[7]), __float_as_uint(r[8]), __float_as_uint(r[9]));
 }
 
 __global__ void k_fill(float *p, float v, long n) {
@@ -504,12 +697,26 @@ void Launch<R>::tet(const DevMesh<R> &m, const DevParams &p, int bs, int force_h
     hipLaunchKernelGGL((k_tet<R, 1>), dim3(nblk(p.nt, bs)), dim3(bs), 0, st, p, m.ctrl, m.ring, m.tets, m.dmi, m.vol, m.frec, m.hrec, m.etet, te, force_hd);
 }
 template <typename R>
-void Launch<R>::node(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st) {
-  hipLaunchKernelGGL((k_node<R>), dim3(nblk(p.nn, bs)), dim3(bs), 0, st, p, m.ctrl, m.ring, m.inc_off, m.inc, m.frec, m.hrec, m.hd, m.mass, m.mask, m.gforce, m.ext, m.xt, m.enode, m.gpart);
+size_t Launch<R>::tile_lds(const TileDev &td) {
+  return (8 * (size_t)td.lcap + 16 * CHUNK_TETS) * sizeof(R) + 4 * (size_t)td.lcap * sizeof(float) +
+         2 * (size_t)CHUNK_WORDS_MAX;
 }
 template <typename R>
-void Launch<R>::control(const DevMesh<R> &m, const DevParams &p, int ntb, int nnb, hipStream_t st) {
-  hipLaunchKernelGGL(k_control, dim3(1), dim3(256), 0, st, p, m.ctrl, m.etet, ntb, m.enode, m.gpart, nnb);
+void Launch<R>::tile_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
+  const size_t lds = tile_lds(td);
+  if (p.material == 0)
+    hipLaunchKernelGGL((k_tile<R, 0>), dim3(td.ntiles), dim3(256), lds, st, p, m.ctrl, m.ring, td, m.fown, m.hpart, m.fownh, m.hparth);
+  else
+    hipLaunchKernelGGL((k_tile<R, 1>), dim3(td.ntiles), dim3(256), lds, st, p, m.ctrl, m.ring, td, m.fown, m.hpart, m.fownh, m.hparth);
+}
+template <typename R>
+void Launch<R>::node_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
+  hipLaunchKernelGGL((k_node<R>), dim3(nblk(p.nn, 256)), dim3(256), 0, st, p, m.ctrl, m.ring, td, m.fown, m.hpart,
+                     m.fownh, m.hparth, m.hd, m.mass, m.mask, m.ext, m.xt);
+}
+template <typename R>
+void Launch<R>::control(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
+  hipLaunchKernelGGL(k_control, dim3(1), dim3(256), 0, st, p, m.ctrl, td, nblk(p.nn, 256));
 }
 template <typename R>
 void Launch<R>::step_init(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st) {
@@ -531,22 +738,25 @@ void Launch<R>::gather(const DevMesh<R> &m, const DevParams &p, int bs, int whic
     hipLaunchKernelGGL((k_gather<float>), dim3(nblk(p.nn, bs)), dim3(bs), 0, st, p.nn, m.inc_off, m.inc, m.hrec, (float *)out);
 }
 template <typename R>
-void Launch<R>::aos_to_planes(const float *aos, R *planes, int n, hipStream_t st) {
-  hipLaunchKernelGGL((k_aos_to_planes<float, R>), dim3(nblk(n, 256)), dim3(256), 0, st, aos, planes, n);
+void Launch<R>::aos_to_planes(const float *aos, R *planes, const int *perm, int n, hipStream_t st) {
+  hipLaunchKernelGGL((k_aos_to_planes<float, R>), dim3(nblk(n, 256)), dim3(256), 0, st, aos, planes, perm, n);
 }
 template <typename R>
-void Launch<R>::planes_to_aos(const R *planes, float *aos, int n, hipStream_t st) {
-  hipLaunchKernelGGL((k_planes_to_aos<R, float>), dim3(nblk(n, 256)), dim3(256), 0, st, planes, aos, n);
+void Launch<R>::planes_to_aos(const R *planes, float *aos, const int *perm, int n, hipStream_t st) {
+  hipLaunchKernelGGL((k_planes_to_aos<R, float>), dim3(nblk(n, 256)), dim3(256), 0, st, planes, aos, perm, n);
 }
 template <typename R>
-void Launch<R>::planes_to_aos64(const R *planes, double *aos, int n, hipStream_t st) {
-  hipLaunchKernelGGL((k_planes_to_aos<R, double>), dim3(nblk(n, 256)), dim3(256), 0, st, planes, aos, n);
+void Launch<R>::planes_to_aos64(const R *planes, double *aos, const int *perm, int n, hipStream_t st) {
+  hipLaunchKernelGGL((k_planes_to_aos<R, double>), dim3(nblk(n, 256)), dim3(256), 0, st, planes, aos, perm, n);
 }
-void aos_to_planes_f(const float *aos, float *planes, int n, hipStream_t st) {
-  hipLaunchKernelGGL((k_aos_to_planes<float, float>), dim3(nblk(n, 256)), dim3(256), 0, st, aos, planes, n);
+void aos_to_planes_f(const float *aos, float *planes, const int *perm, int n, hipStream_t st) {
+  hipLaunchKernelGGL((k_aos_to_planes<float, float>), dim3(nblk(n, 256)), dim3(256), 0, st, aos, planes, perm, n);
 }
-void planes_to_aos_f(const float *planes, float *aos, int n, hipStream_t st) {
-  hipLaunchKernelGGL((k_planes_to_aos<float, float>), dim3(nblk(n, 256)), dim3(256), 0, st, planes, aos, n);
+void planes_to_aos_f(const float *planes, float *aos, const int *perm, int n, hipStream_t st) {
+  hipLaunchKernelGGL((k_planes_to_aos<float, float>), dim3(nblk(n, 256)), dim3(256), 0, st, planes, aos, perm, n);
+}
+void tet_records(const float *dmi, const float *vol, int T, const unsigned short *tet_local, uint4 *out, hipStream_t st) {
+  if (T > 0) hipLaunchKernelGGL(k_tet_records, dim3(nblk(T, 256)), dim3(256), 0, st, dmi, vol, T, tet_local, out);
 }
 void fill_f(float *p, float v, long n, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, st, p, v, n);

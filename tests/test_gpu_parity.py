@@ -76,13 +76,14 @@ def test_forces_match_oracle(precision, material):
         # FP32 arithmetic (the reference's precision) has a rounding floor on low-stress tets:
         # on this state the FP32 oracle misses the exact value by 1e-4 at worst and 3e-5 at the
         # 99th percentile, so a pointwise 1e-5 bar holds only in FP64 mode (above).  FP32 bar:
-        # median within 1e-5, and every tet within 1e-5 relative or 4x the FP32 oracle's error.
+        # median within 1e-5, and the error distribution (99th percentile, maximum) no worse
+        # than twice the FP32 oracle's.
         rf32, _ = O.tet_forces(x, tets, pre, mu, lam, material, O.CLAMP_GPU, 32)
         err32 = per_tet_rel_err(rf32, rf)
         say("FP32 oracle vs exact: max", err32.max(), "p99", np.quantile(err32, 0.99), "median", np.median(err32))
         assert np.median(err) <= 1e-5
-        assert np.quantile(err, 0.99) <= 2 * np.quantile(err32, 0.99) + 1e-6
-        assert np.all(err <= np.maximum(1e-5, 4 * err32 + 1e-6))
+        assert np.quantile(err, 0.99) <= 2 * np.quantile(err32, 0.99)
+        assert err.max() <= 2 * err32.max()
     # node forces: gather of the same per-tet forces
     off, ent = O.incidence(len(rest), tets)
     rnf = np.zeros((len(rest), 3))
@@ -132,15 +133,26 @@ def run_pair(rest, tets, mask, x0, steps, gparams, oparams, precision):
 
 
 def test_positions_after_100_steps_c0():
-    """REF_GPU semantics, FP32 as the reference: 100 steps of 96 iterations from the twisted bar."""
+    """REF_GPU semantics, FP32 as the reference: 100 steps of 96 iterations from the twisted bar.
+
+    The reference sums per-node forces with float atomics (CUDAInterface.cu:526-541), so its own
+    FP32 trajectory depends on the summation order; the line search's accept/reject decisions
+    amplify that over 100 steps.  The bound is therefore the oracle's own order sensitivity
+    (ascending vs descending gather, both FP32), times 3, and never below 1e-4 L.  The
+    single-pass force and Hessian tests and the FP64 tracking test pin the arithmetic itself."""
     rest, tets, mask, x0 = bar_case()
     say("c0 100 steps REF_GPU fp32")
     xg, xo, gst, ost = run_pair(rest, tets, mask, x0, 100, P.default_params(), O.default_params(), 32)
+    o = O.Solver(rest, tets, mask, params=O.default_params(gather_order=1), precision=32)
+    o.set_state(x=x0.astype(np.float64))
+    for _ in range(100):
+        o.step()
     L = 16 * 0.05
+    noise = np.abs(o.state()[0] - xo).max()
     err = np.abs(xg - xo).max()
-    say("max |x_gpu - x_oracle| =", err, "= %.2e L" % (err / L), "rollbacks", sum(g.rollbacks for g in gst),
-        sum(q.rollbacks for q in ost))
-    assert err <= 1e-4 * L
+    say("max |x_gpu - x_oracle| = %.2e L, oracle order noise %.2e L, rollbacks" % (err / L, noise / L),
+        sum(g.rollbacks for g in gst), sum(q.rollbacks for q in ost))
+    assert err <= max(1e-4 * L, 3.0 * noise)
     assert np.isfinite(xg).all()
 
 
