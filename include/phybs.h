@@ -69,8 +69,8 @@ typedef struct pbs_params {
   double relgrad_tol;   /* 0: fixed iterations; > 0: iterate until |g_k| <= tol |g_0| (free DOFs, FP64) */
   int max_iterations;   /* iteration cap in rel-grad mode */
   int block_size;       /* threads per workgroup of the per-tet / per-node utility kernels, 0 = 256 */
-  int tile_nodes;       /* nodes owned per workgroup of the fused descent kernel (64..1024), 0 = by mesh size */
-  int chunk;            /* tets staged in LDS per round of the fused kernel (128, 256, 512), 0 = largest that fits */
+  int tile_tets;        /* max tets per workgroup of the tile pass (16..4096), 0 = chosen from mesh size and LDS */
+  int reserved;         /* must be 0 */
 } pbs_params;
 
 typedef struct pbs_step_stats {

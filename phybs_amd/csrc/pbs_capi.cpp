@@ -92,9 +92,9 @@ static int validate_params(const pbs_params *p) {
   if (p->relgrad_tol < 0 || (p->relgrad_tol > 0 && p->max_iterations <= 0)) return set_error(PBS_ERR_ARG, "bad rel-grad settings");
   if (p->block_size != 0 && p->block_size != 64 && p->block_size != 128 && p->block_size != 256)
     return set_error(PBS_ERR_ARG, "block_size must be 0, 64, 128 or 256");
-  if (p->tile_nodes != 0 && (p->tile_nodes < 16 || p->tile_nodes > 1024))
-    return set_error(PBS_ERR_ARG, "tile_nodes must be 0 or in [16, 1024]");
-  if (p->chunk != 0 && p->chunk != CHUNK_TETS) return set_error(PBS_ERR_ARG, "chunk must be 0 or 256");
+  if (p->tile_tets != 0 && (p->tile_tets < 16 || p->tile_tets > 4096))
+    return set_error(PBS_ERR_ARG, "tile_tets must be 0 or in [16, 4096]");
+  if (p->reserved != 0) return set_error(PBS_ERR_ARG, "reserved must be 0");
   return PBS_OK;
 }
 
@@ -222,14 +222,12 @@ template <typename R> static int alloc_impl(pbs_solver *s) {
   const size_t H = std::max<long>(P.halo_total, 1);
   TileInfo *tiles;
   int *halo, *bnd_off, *bnd;
-  uint4 *trec;
-  uint32_t *coff;
-  unsigned short *cinc;
+  uint32_t *tlp;
+  unsigned short *toff;
   rc |= dalloc(s, &tiles, (size_t)P.ntiles);
   rc |= dalloc(s, &halo, H);
-  rc |= dalloc(s, &trec, 3 * T);
-  rc |= dalloc(s, &coff, P.chunk_off.size());
-  rc |= dalloc(s, &cinc, P.chunk_inc.size() + 8 * CHUNK_WORDS_MAX);  // a prefetch may read one block past the end
+  rc |= dalloc(s, &tlp, 4 * T);
+  rc |= dalloc(s, &toff, P.tile_off.size());
   rc |= dalloc(s, &bnd_off, N + 1);
   rc |= dalloc(s, &bnd, std::max<size_t>(P.bnd.size(), 1));
   rc |= dalloc(s, &s->td.tpart, (size_t)P.ntiles);
@@ -240,9 +238,8 @@ template <typename R> static int alloc_impl(pbs_solver *s) {
   rc |= dalloc(s, &m.hparth, 3 * H);
   s->td.tiles = tiles;
   s->td.halo = halo;
-  s->td.tet_rec = trec;
-  s->td.chunk_off = coff;
-  s->td.chunk_inc = cinc;
+  s->td.tet_lp = (const uint4 *)tlp;
+  s->td.tile_off = toff;
   s->td.bnd_off = bnd_off;
   s->td.bnd = bnd;
   rc |= dalloc(s, &m.ctrl, 1);
@@ -272,10 +269,9 @@ template <typename R> static int setup_impl(pbs_solver *s, const int *mask) {
   HIPCHK(hipMemcpyAsync((void *)s->td.tiles, P.tiles.data(), sizeof(TileInfo) * P.tiles.size(), hipMemcpyHostToDevice, s->st));
   if (!P.halo.empty())
     HIPCHK(hipMemcpyAsync((void *)s->td.halo, P.halo.data(), sizeof(int) * P.halo.size(), hipMemcpyHostToDevice, s->st));
-  HIPCHK(hipMemcpyAsync((void *)s->td.chunk_off, P.chunk_off.data(), sizeof(uint32_t) * P.chunk_off.size(), hipMemcpyHostToDevice, s->st));
-  HIPCHK(hipMemsetAsync((void *)s->td.chunk_inc, 0, sizeof(uint16_t) * (P.chunk_inc.size() + 8 * CHUNK_WORDS_MAX), s->st));
-  if (!P.chunk_inc.empty())
-    HIPCHK(hipMemcpyAsync((void *)s->td.chunk_inc, P.chunk_inc.data(), sizeof(uint16_t) * P.chunk_inc.size(), hipMemcpyHostToDevice, s->st));
+  if (s->nt > 0)
+    HIPCHK(hipMemcpyAsync((void *)s->td.tet_lp, P.tet_lp.data(), sizeof(uint32_t) * P.tet_lp.size(), hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemcpyAsync((void *)s->td.tile_off, P.tile_off.data(), sizeof(uint16_t) * P.tile_off.size(), hipMemcpyHostToDevice, s->st));
   HIPCHK(hipMemcpyAsync((void *)s->td.bnd_off, P.bnd_off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, s->st));
   if (!P.bnd.empty())
     HIPCHK(hipMemcpyAsync((void *)s->td.bnd, P.bnd.data(), sizeof(int) * P.bnd.size(), hipMemcpyHostToDevice, s->st));
@@ -287,18 +283,6 @@ template <typename R> static int setup_impl(pbs_solver *s, const int *mask) {
   HIPCHK(hipGetLastError());
   Launch<R>::setup_nodes(m, s->dp, 256, s->st);
   HIPCHK(hipGetLastError());
-  if (s->nt > 0) {
-    unsigned short *tl = nullptr;
-    HIPCHK(hipMalloc((void **)&tl, sizeof(uint16_t) * P.tet_local.size()));
-    hipError_t e = hipMemcpyAsync(tl, P.tet_local.data(), sizeof(uint16_t) * P.tet_local.size(), hipMemcpyHostToDevice, s->st);
-    if (e == hipSuccess) {
-      tet_records(m.dmi, m.vol, s->nt, tl, (uint4 *)s->td.tet_rec, s->st);
-      e = hipGetLastError();
-    }
-    (void)hipStreamSynchronize(s->st);
-    (void)hipFree(tl);
-    if (e != hipSuccess) return set_error(PBS_ERR_HIP, std::string("tet records: ") + hipGetErrorString(e));
-  }
   int flag = 0;
   HIPCHK(hipMemcpyAsync(&flag, m.flag, sizeof(int), hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
@@ -390,40 +374,28 @@ extern "C" int pbs_create(int nn, const float *nodes, const float *rest, const i
     if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device) != hipSuccess) maxlds = 65536;
     s->lds_budget = (size_t)maxlds;
   }
-  // Tile size: 512 owned nodes, halved for small meshes (to keep >= 512 tiles) and while the tile
-  // pass would need more than 80 KiB of LDS (two workgroups per 160 KiB CU).
-  const size_t rsz = p->precision == PBS_FP64 ? 8 : 4;
-  auto lds_need = [&](int lcap) {
+  // Tile size (tets per tile): 512, halved while the mesh would give fewer than 1024 tiles, and
+  // while the tile pass could need more than 40 KiB of LDS (four workgroups per 160 KiB CU); a
+  // plan with tile_tets = t has at most t tets and max(64, t / 2) local nodes per tile.
+  const bool f64 = p->precision == PBS_FP64;
+  auto lds_need = [&](int lcap, int tcap) {
     TileDev t{};
-    t.lcap = (std::max(lcap, 1) + 3) / 4 * 4;
-    return rsz == 8 ? Launch<double>::tile_lds(t) : Launch<float>::tile_lds(t);
+    t.lcap = std::max(lcap, 1);
+    t.tcap = std::max(tcap, 1);
+    return f64 ? Launch<double>::tile_lds(t) : Launch<float>::tile_lds(t);
   };
-  int tn = p->tile_nodes;
-  const bool auto_tile = tn == 0;
-  if (auto_tile) {
-    tn = 512;
-    while (tn > 64 && nn / tn < 512) tn /= 2;
+  int tt = p->tile_tets;
+  if (tt == 0) {
+    tt = 512;
+    while (tt > 64 && (nt / tt < 1024 || lds_need(std::max(64, tt / 2), tt) > std::min<size_t>(s->lds_budget, 40960))) tt /= 2;
   }
   std::string msg;
-  for (;;) {
-    if (!build_tile_plan(nn, rest, nt, tets, tn, s->plan, msg)) {
-      if (auto_tile && tn > 16) {
-        tn /= 2;
-        continue;
-      }
-      return fail(set_error(PBS_ERR_MESH, msg));
-    }
-    const size_t need = lds_need(s->plan.lcap);
-    if (auto_tile && tn > 16 && need > std::min<size_t>(s->lds_budget, 81920)) {
-      tn /= 2;
-      continue;
-    }
-    if (need > s->lds_budget) return fail(set_error(PBS_ERR_ARG, "tile too large for the LDS of one workgroup: use smaller tile_nodes"));
-    break;
-  }
+  if (!build_tile_plan(nn, rest, nt, tets, tt, s->plan, msg)) return fail(set_error(PBS_ERR_MESH, msg));
+  if (lds_need(s->plan.lcap, s->plan.tcap) > s->lds_budget)
+    return fail(set_error(PBS_ERR_ARG, "tile too large for the LDS of one workgroup: use a smaller tile_tets"));
   s->td.ntiles = s->plan.ntiles;
-  s->td.tile_nodes = tn;
-  s->td.lcap = (std::max(s->plan.lcap, 1) + 3) / 4 * 4;  // keeps the LDS regions after it 16-byte aligned
+  s->td.lcap = std::max(s->plan.lcap, 1);
+  s->td.tcap = std::max(s->plan.tcap, 1);
   s->td.halo_total = s->plan.halo_total;
   if (hipStreamCreate(&s->st) != hipSuccess) return fail(set_error(PBS_ERR_HIP, "hipStreamCreate failed"));
   if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)

@@ -42,18 +42,20 @@ struct Ctrl {
 
 // Tile partition on the device (see pbs_tiles.h for the host-side construction).
 struct TileDev {
-  int ntiles, tile_nodes, lcap, pad;
+  int ntiles, lcap, tcap, pad;
   long halo_total;
   const TileInfo *tiles;
   const int *halo;
-  const uint4 *tet_rec;         // 3 x uint4 per tet (solver order): {local ids 0|1, local ids 2|3,
-                                // Dm^-1[0..1]}, {Dm^-1[2..5]}, {Dm^-1[6..8], volume}
-  const uint32_t *chunk_off;    // per chunk (+1): offset in 16-bit words into chunk_inc
-  const unsigned short *chunk_inc;
+  const uint4 *tet_lp;          // per tet: corner local ids, corner contribution positions (16 bits each)
+  const unsigned short *tile_off;
   const int *bnd_off, *bnd;     // per node: halo partial slots of its force, in tile order
   double *tpart;                // per tile: elastic energy of its tets
   double *npart;                // per node block: node energy, |g|^2
 };
+
+// Values per corner contribution in the tile pass's LDS: 4 floats (one 16-byte access) or 3 doubles.
+template <typename R> struct ContribWidth { static constexpr int value = 3; };
+template <> struct ContribWidth<float> { static constexpr int value = 4; };
 
 template <typename R> struct Ring {
   R *b[4];  // each: 3 planes of nn values (x | y | z)
@@ -108,7 +110,6 @@ template <typename R> struct Launch {
 };
 void aos_to_planes_f(const float *aos, float *planes, const int *perm, int n, hipStream_t st);
 void planes_to_aos_f(const float *planes, float *aos, const int *perm, int n, hipStream_t st);
-void tet_records(const float *dmi, const float *vol, int T, const unsigned short *tet_local, uint4 *out, hipStream_t st);
 void fill_f(float *p, float v, long n, hipStream_t st);
 
 }  // namespace pbs

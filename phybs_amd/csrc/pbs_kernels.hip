@@ -10,8 +10,8 @@
 //
 // Layout: nodes are renumbered in Morton order and tets sorted by their smallest node; node
 // vectors are 3 planes (x | y | z) of nn values (SoA); per-tet rest data are 10 planes (Dm^-1
-// entries, volume) plus one 48-byte record per tet for the tile pass; per-tet force records of
-// the standalone calls are 12 consecutive values.
+// entries, volume) plus 16 bytes per tet of tile-local ids and contribution slots for the tile
+// pass; per-tet force records of the standalone calls are 12 consecutive values.
 #include "pbs_device.h"
 #include "pbs_internal.h"
 
@@ -248,16 +248,19 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblk) {
 // ---------------------------------------------------------------------------------------------
 // Tile pass (per-tet half of computeForceKd, CUDAInterface.cu:495-568): one workgroup per tile
 // evaluates each of the tile's tets once -- force, energy on check passes, Hessian diagonal on
-// refresh passes -- and gathers the results per local node in LDS in a fixed order (no atomics).
-// Outputs: a partial sum per owned node (fown) and per halo node (hpart).
-//   LDS: positions of local nodes (4 R each), force sums (4 R), Hessian sums (4 float), one
-//   chunk of per-tet records (256 tets x 4 corners x 4 R), one chunk's incidence block (8 KiB).
-//   The 48-byte records and the incidence block of chunk c+1 are loaded while chunk c computes.
+// refresh passes -- and writes every corner contribution into its precomputed slot of an LDS
+// array in which each local node's contributions form one contiguous run (ordered by tet, then
+// corner).  The gather then sums each run in that fixed order (no atomics) and writes one partial
+// per owned node (fown) and per halo node (hpart).
+//   LDS: positions of local nodes (4 R each), contributions (4 per tet, W values each).
+//   HBM per tet: 16 B of local ids / slots, 40 B of rest data from the SoA planes.
 // ---------------------------------------------------------------------------------------------
 template <typename R, int MAT>
 __global__ void __launch_bounds__(256)
-k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, R *__restrict__ fown,
-       R *__restrict__ hpart, float *__restrict__ fownh, float *__restrict__ hparth) {
+k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, const float *__restrict__ dmi,
+       const float *__restrict__ vol, R *__restrict__ fown, R *__restrict__ hpart, float *__restrict__ fownh,
+       float *__restrict__ hparth) {
+  constexpr int W = ContribWidth<R>::value;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double lds_red[16];
   if (ctrl->done) return;
@@ -265,145 +268,107 @@ k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, R *
   const R *__restrict__ x = ring.b[ctrl->cur];
   const int b = xcd_tile(blockIdx.x, gridDim.x);
   const TileInfo ti = td.tiles[b];
-  const int lcap = td.lcap, L = ti.own_cnt + ti.halo_cnt, N = p.nn, tid = threadIdx.x;
+  const int L = ti.own_cnt + ti.halo_cnt, N = p.nn, T = p.nt, tid = threadIdx.x;
   const long H = td.halo_total;
   R *sx = (R *)smem;
-  R *facc = sx + 4 * lcap;
-  float *hacc = (float *)(facc + 4 * lcap);
-  R *rec = (R *)(hacc + 4 * lcap);
-  unsigned short *sinc = (unsigned short *)(rec + CHUNK_TETS * 16);
+  R *rec = sx + 4 * td.lcap;
   for (int i = tid; i < L; i += 256) {
     const int g = i < ti.own_cnt ? ti.own_start + i : td.halo[ti.halo_off + i - ti.own_cnt];
     sx[4 * i] = x[g];
     sx[4 * i + 1] = x[N + g];
     sx[4 * i + 2] = x[2 * (size_t)N + g];
-#pragma unroll
-    for (int d = 0; d < 4; ++d) {
-      facc[4 * i + d] = (R)0;
-      hacc[4 * i + d] = 0.0f;
-    }
   }
-  uint4 nr[3], ni[2] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
-  auto prefetch = [&](int c) {
-    if (c < ti.nchunk) {
-      const int t = ti.tet_start + c * CHUNK_TETS + tid;
-      if (c * CHUNK_TETS + tid < ti.tet_cnt) {
-        const uint4 *r = td.tet_rec + 3 * (size_t)t;
-        nr[0] = r[0];
-        nr[1] = r[1];
-        nr[2] = r[2];
-      }
-      const uint32_t o0 = td.chunk_off[ti.chunk_first + c], o1 = td.chunk_off[ti.chunk_first + c + 1];
-      const uint4 *src = (const uint4 *)(td.chunk_inc + o0);
-      const int nv = (int)(o1 - o0) >> 3;
-      if (tid < nv) ni[0] = src[tid];
-      if (tid + 256 < nv) ni[1] = src[tid + 256];
-    }
-  };
-  prefetch(0);
   __syncthreads();
   const R mu = (R)p.mu, lam = (R)p.lam;
   double e_el = 0.0;
-  for (int c = 0; c < ti.nchunk; ++c) {
-    ((uint4 *)sinc)[tid] = ni[0];
-    ((uint4 *)sinc)[tid + 256] = ni[1];
-    const uint4 r0 = nr[0], r1 = nr[1], r2 = nr[2];
-    const bool have = c * CHUNK_TETS + tid < ti.tet_cnt;
-    prefetch(c + 1);
-    for (int phase = 0; phase <= hdnow; ++phase) {
-      if (have) {
-        const int a0 = r0.x & 0xffff, a1 = r0.x >> 16, a2 = r0.y & 0xffff, a3 = r0.y >> 16;
-        R Dmi[9];
-        Dmi[0] = (R)__uint_as_float(r0.z);
-        Dmi[1] = (R)__uint_as_float(r0.w);
-        Dmi[2] = (R)__uint_as_float(r1.x);
-        Dmi[3] = (R)__uint_as_float(r1.y);
-        Dmi[4] = (R)__uint_as_float(r1.z);
-        Dmi[5] = (R)__uint_as_float(r1.w);
-        Dmi[6] = (R)__uint_as_float(r2.x);
-        Dmi[7] = (R)__uint_as_float(r2.y);
-        Dmi[8] = (R)__uint_as_float(r2.z);
-        const R V = (R)__uint_as_float(r2.w);
-        const R *p0 = sx + 4 * a0, *p1 = sx + 4 * a1, *p2 = sx + 4 * a2, *p3 = sx + 4 * a3;
-        R Ds[9];
+  for (int phase = 0; phase <= hdnow; ++phase) {
+    for (int s = tid; s < ti.tet_cnt; s += 256) {
+      const size_t t = (size_t)ti.tet_start + s;
+      const uint4 lp = td.tet_lp[t];
+      R Dmi[9];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const R x0 = p0[k];
-          Ds[3 * k + 0] = p1[k] - x0;
-          Ds[3 * k + 1] = p2[k] - x0;
-          Ds[3 * k + 2] = p3[k] - x0;
+      for (int k = 0; k < 9; ++k) Dmi[k] = (R)dmi[k * (size_t)T + t];
+      const R V = (R)vol[t];
+      const R *p0 = sx + 4 * (lp.x & 0xffff), *p1 = sx + 4 * (lp.x >> 16), *p2 = sx + 4 * (lp.y & 0xffff),
+              *p3 = sx + 4 * (lp.y >> 16);
+      R Ds[9];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const R x0 = p0[k];
+        Ds[3 * k + 0] = p1[k] - x0;
+        Ds[3 * k + 1] = p2[k] - x0;
+        Ds[3 * k + 2] = p3[k] - x0;
+      }
+      R F[9];
+      mul3(Ds, Dmi, F);
+      R f12[12], h12[12], en;
+      const unsigned q[4] = {lp.z & 0xffff, lp.z >> 16, lp.w & 0xffff, lp.w >> 16};
+      if (phase == 0) {
+        if (MAT == 0) neo_tet<R, false>(F, Dmi, V, mu, lam, f12, en, h12);
+        else stretch_tet<R, false>(F, Dmi, V, mu, lam, f12, en, h12);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          R *r = rec + W * q[c];
+          r[0] = f12[3 * c];
+          r[1] = f12[3 * c + 1];
+          r[2] = f12[3 * c + 2];
         }
-        R F[9];
-        mul3(Ds, Dmi, F);
-        R f12[12], h12[12], en;
-        R *rr = rec + 16 * tid;
-        if (phase == 0) {
-          if (MAT == 0) neo_tet<R, false>(F, Dmi, V, mu, lam, f12, en, h12);
-          else stretch_tet<R, false>(F, Dmi, V, mu, lam, f12, en, h12);
+        if (chk) {
+          if (p.clamp == 1) en = en > (R)0 ? en : (R)0;          // CUDAInterface.cu:549
+          else if (p.clamp == 2) en = en > (R)1e-6 ? en : (R)0;  // IsotropicNeohookeanMaterial.cpp:129
+          e_el += (double)en;
+        }
+      } else {
+        if (MAT == 0) neo_tet<R, true>(F, Dmi, V, mu, lam, f12, en, h12);
+        else stretch_tet<R, true>(F, Dmi, V, mu, lam, f12, en, h12);
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            rr[4 * q] = f12[3 * q];
-            rr[4 * q + 1] = f12[3 * q + 1];
-            rr[4 * q + 2] = f12[3 * q + 2];
-          }
-          if (chk) {
-            if (p.clamp == 1) en = en > (R)0 ? en : (R)0;          // CUDAInterface.cu:549
-            else if (p.clamp == 2) en = en > (R)1e-6 ? en : (R)0;  // IsotropicNeohookeanMaterial.cpp:129
-            e_el += (double)en;
-          }
-        } else {
-          if (MAT == 0) neo_tet<R, true>(F, Dmi, V, mu, lam, f12, en, h12);
-          else stretch_tet<R, true>(F, Dmi, V, mu, lam, f12, en, h12);
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            rr[4 * q] = (R)(float)h12[3 * q];
-            rr[4 * q + 1] = (R)(float)h12[3 * q + 1];
-            rr[4 * q + 2] = (R)(float)h12[3 * q + 2];
-          }
+        for (int c = 0; c < 4; ++c) {
+          R *r = rec + W * q[c];
+          r[0] = (R)(float)h12[3 * c];
+          r[1] = (R)(float)h12[3 * c + 1];
+          r[2] = (R)(float)h12[3 * c + 2];
         }
       }
-      __syncthreads();
-      const int nn = sinc[0];
-      const unsigned short *nodes = sinc + 1, *offs = sinc + 1 + nn, *ents = sinc + 2 + 2 * nn;
-      for (int i = tid; i < nn; i += 256) {
-        const int node = nodes[i];
-        R a0s = (R)0, a1s = (R)0, a2s = (R)0;
-        for (int e = offs[i]; e < offs[i + 1]; ++e) {
-          const int v = ents[e];
-          const R *rr = rec + 16 * (v >> 2) + 4 * (v & 3);
-          a0s += rr[0];
-          a1s += rr[1];
-          a2s += rr[2];
-        }
-        if (phase == 0) {
-          facc[4 * node] += a0s;
-          facc[4 * node + 1] += a1s;
-          facc[4 * node + 2] += a2s;
-        } else {
-          hacc[4 * node] += (float)a0s;
-          hacc[4 * node + 1] += (float)a1s;
-          hacc[4 * node + 2] += (float)a2s;
-        }
+    }
+    __syncthreads();
+    const unsigned short *off = td.tile_off + ti.off_start;
+    for (int i = tid; i < L; i += 256) {
+      const int o0 = off[i], o1 = off[i + 1];
+      size_t d0, stride;
+      if (i < ti.own_cnt) {
+        d0 = (size_t)ti.own_start + i;
+        stride = N;
+      } else {
+        d0 = (size_t)ti.halo_off + (i - ti.own_cnt);
+        stride = H;
       }
-      __syncthreads();
+      if (phase == 0) {
+        R a0 = (R)0, a1 = (R)0, a2 = (R)0;
+        for (int j = o0; j < o1; ++j) {
+          const R *r = rec + W * j;
+          a0 += r[0];
+          a1 += r[1];
+          a2 += r[2];
+        }
+        R *o = i < ti.own_cnt ? fown : hpart;
+        o[d0] = a0;
+        o[stride + d0] = a1;
+        o[2 * stride + d0] = a2;
+      } else {
+        float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+        for (int j = o0; j < o1; ++j) {
+          const R *r = rec + W * j;
+          a0 += (float)r[0];
+          a1 += (float)r[1];
+          a2 += (float)r[2];
+        }
+        float *o = i < ti.own_cnt ? fownh : hparth;
+        o[d0] = a0;
+        o[stride + d0] = a1;
+        o[2 * stride + d0] = a2;
+      }
     }
-  }
-  for (int i = tid; i < L; i += 256) {
-    if (i < ti.own_cnt) {
-      const size_t g = (size_t)ti.own_start + i;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) fown[d * (size_t)N + g] = facc[4 * i + d];
-      if (hdnow)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) fownh[d * (size_t)N + g] = hacc[4 * i + d];
-    } else {
-      const size_t h = (size_t)ti.halo_off + (i - ti.own_cnt);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) hpart[d * H + h] = facc[4 * i + d];
-      if (hdnow)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) hparth[d * H + h] = hacc[4 * i + d];
-    }
+    if (phase == 0 && hdnow) __syncthreads();  // the Hessian phase reuses the contribution slots
   }
   if (chk) {
     const double se = block_sum(e_el, lds_red);
@@ -654,21 +619,6 @@ __global__ void k_planes_to_aos(const A *__restrict__ planes, B *__restrict__ ao
   for (int d = 0; d < 3; ++d) aos[o + d] = (B)planes[(size_t)d * n + i];
 }
 
-// Per-tet 48-byte record: tile-local node ids, then the tet's Dm^-1 (row-major) and volume.
-__global__ void k_tet_records(const float *__restrict__ dmi, const float *__restrict__ vol, int T,
-                              const unsigned short *__restrict__ loc, uint4 *__restrict__ out) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= T) return;
-  float r[10];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) r[k] = dmi[(size_t)k * T + t];
-  r[9] = vol[t];
-  const unsigned l0 = loc[4 * (size_t)t], l1 = loc[4 * (size_t)t + 1], l2 = loc[4 * (size_t)t + 2], l3 = loc[4 * (size_t)t + 3];
-  out[3 * (size_t)t + 0] = make_uint4(l0 | (l1 << 16), l2 | (l3 << 16), __float_as_uint(r[0]), __float_as_uint(r[1]));
-  out[3 * (size_t)t + 1] = make_uint4(__float_as_uint(r[2]), __float_as_uint(r[3]), __float_as_uint(r[4]), __float_as_uint(r[5]));
-  out[3 * (size_t)t + 2] = make_uint4(__float_as_uint(r[6]), __float_as_uint(r[7]), __float_as_uint(r[8]), __float_as_uint(r[9]));
-}
-
 __global__ void k_fill(float *p, float v, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -698,16 +648,17 @@ void Launch<R>::tet(const DevMesh<R> &m, const DevParams &p, int bs, int force_h
 }
 template <typename R>
 size_t Launch<R>::tile_lds(const TileDev &td) {
-  return (8 * (size_t)td.lcap + 16 * CHUNK_TETS) * sizeof(R) + 4 * (size_t)td.lcap * sizeof(float) +
-         2 * (size_t)CHUNK_WORDS_MAX;
+  return (4 * (size_t)td.lcap + 4 * (size_t)ContribWidth<R>::value * td.tcap) * sizeof(R);
 }
 template <typename R>
 void Launch<R>::tile_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
   const size_t lds = tile_lds(td);
   if (p.material == 0)
-    hipLaunchKernelGGL((k_tile<R, 0>), dim3(td.ntiles), dim3(256), lds, st, p, m.ctrl, m.ring, td, m.fown, m.hpart, m.fownh, m.hparth);
+    hipLaunchKernelGGL((k_tile<R, 0>), dim3(td.ntiles), dim3(256), lds, st, p, m.ctrl, m.ring, td, m.dmi, m.vol, m.fown,
+                       m.hpart, m.fownh, m.hparth);
   else
-    hipLaunchKernelGGL((k_tile<R, 1>), dim3(td.ntiles), dim3(256), lds, st, p, m.ctrl, m.ring, td, m.fown, m.hpart, m.fownh, m.hparth);
+    hipLaunchKernelGGL((k_tile<R, 1>), dim3(td.ntiles), dim3(256), lds, st, p, m.ctrl, m.ring, td, m.dmi, m.vol, m.fown,
+                       m.hpart, m.fownh, m.hparth);
 }
 template <typename R>
 void Launch<R>::node_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
@@ -755,9 +706,7 @@ void aos_to_planes_f(const float *aos, float *planes, const int *perm, int n, hi
 void planes_to_aos_f(const float *planes, float *aos, const int *perm, int n, hipStream_t st) {
   hipLaunchKernelGGL((k_planes_to_aos<float, float>), dim3(nblk(n, 256)), dim3(256), 0, st, planes, aos, perm, n);
 }
-void tet_records(const float *dmi, const float *vol, int T, const unsigned short *tet_local, uint4 *out, hipStream_t st) {
-  if (T > 0) hipLaunchKernelGGL(k_tet_records, dim3(nblk(T, 256)), dim3(256), 0, st, dmi, vol, T, tet_local, out);
-}
+
 void fill_f(float *p, float v, long n, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, st, p, v, n);
 }

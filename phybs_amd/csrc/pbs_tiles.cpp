@@ -1,5 +1,5 @@
-// pbs_tiles.cpp -- node renumbering (Morton order of the rest positions) and the node-tile
-// partition used by the fused descent kernel.  Host-side, run once per solver.
+// pbs_tiles.cpp -- node renumbering (Morton order of the rest positions), tet ordering and the
+// tile partition used by the tile pass of the descent.  Host-side, run once per solver.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -19,12 +19,16 @@ static inline uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
   return v;
 }
 
-bool build_tile_plan(int nn, const float *rest, int nt, const int *tets, int tile_nodes, TilePlan &P,
+bool build_tile_plan(int nn, const float *rest, int nt, const int *tets, int tile_tets, TilePlan &P,
                      std::string &msg) {
   P = TilePlan();
   P.nn = nn;
   P.nt = nt;
-  P.tile_nodes = tile_nodes;
+  P.tile_tets = tile_tets;
+  if (tile_tets < 1 || tile_tets > 16383) {
+    msg = "tile_tets must be in [1, 16383]";
+    return false;
+  }
   // ---- node order: Morton code of positions quantised to the shortest tet edge ----
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
   for (int i = 0; i < nn; ++i)
@@ -84,34 +88,74 @@ bool build_tile_plan(int nn, const float *rest, int nt, const int *tets, int til
         P.inc[P.inc_off[n] + fill[n]++] = 4 * P.itperm[t] + j;
       }
   }
-  // ---- tiles: owned nodes, owned tets (smallest node owned), halo nodes, chunk incidence ----
-  P.ntiles = (nn + tile_nodes - 1) / tile_nodes;
-  P.tiles.resize(P.ntiles);
-  P.tet_local.assign(4 * (size_t)nt, 0);
+  // ---- tiles: owned node runs cut at tile_tets owned tets, at max(64, tile_tets / 2) local
+  // nodes (owned + halo) and at kMaxOwn owned nodes (which bounds the LDS of meshes with many
+  // tet-less nodes); halo nodes; contribution runs ----
+  constexpr int kMaxOwn = 1024;
+  std::vector<int> own_tets(nn + 1, 0);  // tets per smallest node
+  for (int t = 0; t < nt; ++t) own_tets[tkey[t]]++;
+  P.tet_lp.assign(4 * (size_t)nt, 0);
   std::vector<int> loc(nn, -1);
   std::vector<std::vector<int>> bnd_lists(nn);
-  int t0 = 0;
-  for (int b = 0; b < P.ntiles; ++b) {
-    TileInfo &ti = P.tiles[b];
-    ti.own_start = b * tile_nodes;
-    ti.own_cnt = std::min(tile_nodes, nn - ti.own_start);
-    const int own_end = ti.own_start + ti.own_cnt;
-    int t1 = t0;
-    while (t1 < nt && tkey[P.tperm[t1]] < own_end) ++t1;
+  std::vector<uint16_t> lid(4 * (size_t)std::max(tile_tets, 1));
+  std::vector<std::pair<int, int>> ent;  // (local node, 4*slot + corner)
+  std::vector<int> first_tet(nn + 1, 0);  // tets with smallest node n: [first_tet[n], first_tet[n+1])
+  for (int n = 0; n < nn; ++n) first_tet[n + 1] = first_tet[n] + own_tets[n];
+  // local node cap: tiles whose tets reach unusually many distinct nodes are cut short
+  const int lmax = std::max(64, tile_tets / 2);
+  std::vector<int> stamp(nn, -1);
+  int n0 = 0, t0 = 0, ntile = 0;
+  while (n0 < nn) {
+    int n1 = n0, cnt = 0, nloc = 0;
+    while (n1 < nn) {
+      if (n1 > n0 && (cnt + own_tets[n1] > tile_tets || n1 - n0 >= kMaxOwn)) break;
+      // distinct nodes that adding n1 brings (n1 itself and its tets' corners)
+      int add = 0;
+      const int tag = 2 * ntile + 1;  // provisional mark; committed marks are 2 * ntile
+      auto probe = [&](int n) {
+        if (stamp[n] != 2 * ntile && stamp[n] != tag) {
+          stamp[n] = tag;
+          ++add;
+        }
+      };
+      probe(n1);
+      for (int t = first_tet[n1]; t < first_tet[n1 + 1]; ++t)
+        for (int j = 0; j < 4; ++j) probe(P.tets_new[4 * (size_t)t + j]);
+      const bool take = n1 == n0 || nloc + add <= lmax;
+      auto settle = [&](int n) {
+        if (stamp[n] == tag) stamp[n] = take ? 2 * ntile : -1;
+      };
+      settle(n1);
+      for (int t = first_tet[n1]; t < first_tet[n1 + 1]; ++t)
+        for (int j = 0; j < 4; ++j) settle(P.tets_new[4 * (size_t)t + j]);
+      if (!take) break;
+      nloc += add;
+      cnt += own_tets[n1++];
+    }
+    ++ntile;
+    if (cnt > tile_tets) {
+      msg = "a node is the smallest node of more tets than tile_tets";
+      return false;
+    }
+    TileInfo ti{};
+    ti.own_start = n0;
+    ti.own_cnt = n1 - n0;
     ti.tet_start = t0;
-    ti.tet_cnt = t1 - t0;
+    ti.tet_cnt = cnt;
+    const int t1 = t0 + cnt;
     ti.halo_off = (int)P.halo.size();
     std::vector<int> hl;
     for (int t = t0; t < t1; ++t)
       for (int j = 0; j < 4; ++j) {
         int n = P.tets_new[4 * (size_t)t + j];
-        if (n >= own_end) hl.push_back(n);
+        if (n >= n1) hl.push_back(n);
       }
     std::sort(hl.begin(), hl.end());
     hl.erase(std::unique(hl.begin(), hl.end()), hl.end());
     ti.halo_cnt = (int)hl.size();
-    if (ti.own_cnt + ti.halo_cnt > 65535) {
-      msg = "tile has more than 65535 local nodes: use a smaller tile";
+    const int L = ti.own_cnt + ti.halo_cnt;
+    if (L + 1 > 65535) {
+      msg = "tile has more than 65534 local nodes: use smaller tiles";
       return false;
     }
     for (int k = 0; k < ti.halo_cnt; ++k) {
@@ -119,45 +163,41 @@ bool build_tile_plan(int nn, const float *rest, int nt, const int *tets, int til
       P.halo.push_back(hl[k]);
       bnd_lists[hl[k]].push_back(ti.halo_off + k);
     }
+    ent.clear();
     for (int t = t0; t < t1; ++t)
       for (int j = 0; j < 4; ++j) {
         int n = P.tets_new[4 * (size_t)t + j];
-        P.tet_local[4 * (size_t)t + j] = (uint16_t)(n < own_end ? n - ti.own_start : loc[n]);
+        int l = n < n1 ? n - n0 : loc[n];
+        lid[4 * (size_t)(t - t0) + j] = (uint16_t)l;
+        ent.push_back({l, 4 * (t - t0) + j});
       }
-    // incidence blocks per chunk of CHUNK_TETS tets
-    ti.chunk_first = (int)P.chunk_off.size();
-    ti.nchunk = (ti.tet_cnt + CHUNK_TETS - 1) / CHUNK_TETS;
-    std::vector<std::pair<int, int>> ent;  // (local node, slot<<2|corner)
-    for (int c = 0; c < ti.nchunk; ++c) {
-      ent.clear();
-      const int cs = t0 + c * CHUNK_TETS, ce = std::min(t1, cs + CHUNK_TETS);
-      for (int t = cs; t < ce; ++t)
-        for (int j = 0; j < 4; ++j) ent.push_back({P.tet_local[4 * (size_t)t + j], ((t - cs) << 2) | j});
-      std::sort(ent.begin(), ent.end());
-      std::vector<uint16_t> nodes, offs;
-      for (size_t i = 0; i < ent.size(); ++i)
-        if (i == 0 || ent[i].first != ent[i - 1].first) {
-          nodes.push_back((uint16_t)ent[i].first);
-          offs.push_back((uint16_t)i);
-        }
-      offs.push_back((uint16_t)ent.size());
-      const size_t start = P.chunk_inc.size();
-      P.chunk_off.push_back((uint32_t)start);
-      P.chunk_inc.push_back((uint16_t)nodes.size());
-      for (uint16_t v : nodes) P.chunk_inc.push_back(v);
-      for (uint16_t v : offs) P.chunk_inc.push_back(v);
-      for (auto &e : ent) P.chunk_inc.push_back((uint16_t)e.second);
-      while (P.chunk_inc.size() % 8) P.chunk_inc.push_back(0);
-      if (P.chunk_inc.size() - start > (size_t)CHUNK_WORDS_MAX) {
-        msg = "chunk incidence block exceeds 8 KiB";
-        return false;
+    std::sort(ent.begin(), ent.end());
+    ti.off_start = (int)P.tile_off.size();
+    {
+      size_t e = 0;
+      for (int i = 0; i <= L; ++i) {
+        while (e < ent.size() && ent[e].first < i) ++e;
+        P.tile_off.push_back((uint16_t)e);
       }
     }
+    std::vector<uint16_t> pos(4 * (size_t)cnt);
+    for (size_t e = 0; e < ent.size(); ++e) pos[ent[e].second] = (uint16_t)e;
+    for (int t = t0; t < t1; ++t) {
+      const uint16_t *l = &lid[4 * (size_t)(t - t0)], *q = &pos[4 * (size_t)(t - t0)];
+      uint32_t *w = &P.tet_lp[4 * (size_t)t];
+      w[0] = (uint32_t)l[0] | ((uint32_t)l[1] << 16);
+      w[1] = (uint32_t)l[2] | ((uint32_t)l[3] << 16);
+      w[2] = (uint32_t)q[0] | ((uint32_t)q[1] << 16);
+      w[3] = (uint32_t)q[2] | ((uint32_t)q[3] << 16);
+    }
     for (int k = 0; k < ti.halo_cnt; ++k) loc[hl[k]] = -1;
-    P.lcap = std::max(P.lcap, ti.own_cnt + ti.halo_cnt);
+    P.lcap = std::max(P.lcap, L);
+    P.tcap = std::max(P.tcap, cnt);
+    P.tiles.push_back(ti);
+    n0 = n1;
     t0 = t1;
   }
-  P.chunk_off.push_back((uint32_t)P.chunk_inc.size());  // end of the last block
+  P.ntiles = (int)P.tiles.size();
   P.halo_total = (long)P.halo.size();
   P.bnd_off.assign(nn + 1, 0);
   for (int n = 0; n < nn; ++n) P.bnd_off[n + 1] = P.bnd_off[n] + (int)bnd_lists[n].size();
