@@ -38,20 +38,22 @@ def algorithmic_bytes(nt, nn, rsz):
     return {"k_tile": 56.0 * nt + 6.0 * rsz * nn, "k_node": (15.0 * rsz + 20.0) * nn}
 
 
-def cpu_baseline(rest, tets, mask, material, iters_per_step, grav, seconds=12.0):
+def cpu_baseline(rest, tets, mask, material, iters_per_step, grav, seconds=15.0):
     """The oracle (FP64, OpenMP) timed on the host on a bounded sample: a fixed number of descent
-    iterations of the same mesh, scaled to steps/s with the GPU's iterations per step."""
+    iterations of the same mesh, scaled to steps/s with the GPU's iterations per step.  The
+    sample size comes from a 16-iteration probe (which, like the sample, includes one Hessian
+    refresh per 32 iterations at most)."""
     from oracle import pyoracle as O
 
     cores = min(16, len(os.sched_getaffinity(0)))
-    p = O.default_params(iterations=2, grav=grav, dt=1.0 / 30.0,
+    p = O.default_params(iterations=16, grav=grav, dt=1.0 / 30.0,
                          profile=[(1, 0.96), (7, 0.991), (10, 0.9995)])
     o = O.Solver(rest, tets, mask, params=p, precision=64, material=material, threads=cores)
     t0 = time.perf_counter()
     o.step()
     t1 = time.perf_counter()
-    per_iter = (t1 - t0) / 2.0
-    n = max(2, min(200, int(seconds / max(per_iter, 1e-9))))
+    per_iter = (t1 - t0) / 16.0
+    n = max(16, min(400, int(seconds / max(per_iter, 1e-9))))
     p.iterations = n
     o.set_params(p)
     o.set_state(x=rest.astype(np.float64))
@@ -72,6 +74,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2a", choices=sorted(WORKLOADS))
     ap.add_argument("--mode", default="relgrad", choices=["relgrad", "fixed"])
+    ap.add_argument("--precision", default="auto", choices=["auto", "f32", "f64"],
+                    help="state and per-tet arithmetic (auto: f64 for rel-grad, f32 for fixed iterations)")
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--max-iterations", type=int, default=40000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -105,15 +109,16 @@ def main():
     x0 = P.mesh_twist(rest, nx, ny, nz, edge, twist, jitter, 1) if (twist or jitter) else rest
     grav = (0.0, 0.0, -9.8)
     relg = args.mode == "relgrad"
+    f64 = args.precision == "f64" or (args.precision == "auto" and relg)
     params = P.default_params(
-        dt=1.0 / 30.0, grav=grav, material=material, precision=P.FP64 if relg else P.FP32,
+        dt=1.0 / 30.0, grav=grav, material=material, precision=P.FP64 if f64 else P.FP32,
         iterations=fixed_iters, relgrad_tol=args.tol if relg else 0.0, max_iterations=args.max_iterations,
         profile=[(1, 0.96), (7, 0.991), (10, 0.9995)])
     s = P.Solver(rest, tets, mask, params)
     s.set_positions(x0)
     for _ in range(args.warmup):
         s.step()
-    s.set_profiling(True)
+    s.set_profiling(8)  # HIP events around the kernels of every 8th graph replay (32 passes each)
     s.reset_kernel_times()
     stats = []
     t0 = time.perf_counter()
@@ -125,7 +130,7 @@ def main():
     iters = sum(st.iterations for st in stats)
     passes = sum(st.passes for st in stats)
     nt, nn = len(tets), len(rest)
-    alg = algorithmic_bytes(nt, nn, 8 if relg else 4)
+    alg = algorithmic_bytes(nt, nn, 8 if f64 else 4)
     dom = max(("k_tile", "k_node"), key=lambda k: kt[k][0] * kt[k][1])
     avg_ms = kt[dom][0]
     achieved = alg[dom] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
@@ -141,7 +146,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if relg else "f32",
+        "dtype": "f64" if f64 else "f32",
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {nx}x{ny}x{nz} bar, {nt} tets, {nn} nodes, "
                                f"{'stretch' if material else 'neo-Hookean'}, "

@@ -70,7 +70,8 @@ typedef struct pbs_params {
   int max_iterations;   /* iteration cap in rel-grad mode */
   int block_size;       /* threads per workgroup of the per-tet / per-node utility kernels, 0 = 256 */
   int tile_tets;        /* max tets per workgroup of the tile pass (16..4096), 0 = chosen from mesh size and LDS */
-  int reserved;         /* must be 0 */
+  int graph_passes;     /* descent passes per captured HIP graph replay: 0 = 32, -1 = no graph
+                           (kernels launched one by one) */
 } pbs_params;
 
 typedef struct pbs_step_stats {
@@ -150,7 +151,10 @@ int pbs_compute_hessian_diag(pbs_solver *s, const float *pos, float *node_hd, fl
  * HIP events on the handle's stream: names[i] / avg_ms[i] / launches[i] for i < *count. */
 int pbs_kernel_times(pbs_solver *s, int max, const char **names, double *avg_ms, long *launches, int *count);
 int pbs_reset_kernel_times(pbs_solver *s);
-int pbs_set_profiling(pbs_solver *s, int on);
+/* period 0: no timing (default); 1: every descent-pass kernel is timed; n > 1: the kernels of
+ * every n-th graph replay (every n-th pass without graphs) are timed.  Launches that run after
+ * the loop has stopped within a replay are not counted. */
+int pbs_set_profiling(pbs_solver *s, int period);
 
 /* ---- host mesh utilities (no GPU work) ---- */
 /* Bar of nx*ny*nz cubes of size edge, 6 tets per cube (Freudenthal split, det(Dm) > 0).

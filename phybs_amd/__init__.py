@@ -34,7 +34,7 @@ class Params(C.Structure):
         ("profile_n", C.c_int), ("profile_k", C.c_int * 8), ("profile_rho", C.c_float * 8),
         ("semantics", C.c_int), ("precision", C.c_int), ("material", C.c_int),
         ("iterations", C.c_int), ("relgrad_tol", C.c_double), ("max_iterations", C.c_int),
-        ("block_size", C.c_int), ("tile_tets", C.c_int), ("reserved", C.c_int),
+        ("block_size", C.c_int), ("tile_tets", C.c_int), ("graph_passes", C.c_int),
     ]
 
 
@@ -277,8 +277,9 @@ class Solver:
         _check(lib().pbs_get_rest_data(self.h, _p(dminv), _p(vol), _p(mass), _p(gf)))
         return dict(dminv=dminv, vol=vol, mass=mass, gforce=gf)
 
-    def set_profiling(self, on=True):
-        _check(lib().pbs_set_profiling(self.h, 1 if on else 0))
+    def set_profiling(self, period=1):
+        """period: 0/False off, 1/True every pass, n: every n-th graph replay."""
+        _check(lib().pbs_set_profiling(self.h, int(period)))
 
     def reset_kernel_times(self):
         _check(lib().pbs_reset_kernel_times(self.h))

@@ -57,10 +57,24 @@ struct pbs_solver {
   double *stage64 = nullptr;  // 3N double AoS staging
   std::vector<void *> allocs;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  bool profiling = false;
   std::vector<hipEvent_t> pool;
-  std::vector<std::pair<int, int>> pending;  // (kernel id, first event index)
+  struct Pending {
+    int kid, ev;  // kernel id, first of its two events
+    long pass;    // pass ordinal within the step (-1: not a descent pass)
+  };
+  std::vector<Pending> pending;
   int pool_used = 0;
+  long pass_ordinal = -1;  // ordinal given to the descent-pass launches being recorded
+  // HIP graphs of graph_passes descent passes, replayed by pbs_step: [0] plain, [1] with
+  // timing events around each kernel (6 per pass, gev)
+  struct PassGraph {
+    hipGraphExec_t exec = nullptr;
+    DevParams dp{};  // parameters baked into the graph
+    int passes = 0;
+  } graph[2];
+  std::vector<hipEvent_t> gev;
+  int prof_period = 0;  // 0: no timing; n: time every n-th graph replay (or pass without graphs)
+  long prof_tick = 0;
   double kt_ms[K_COUNT] = {0};
   long kt_n[K_COUNT] = {0};
 };
@@ -94,7 +108,7 @@ static int validate_params(const pbs_params *p) {
     return set_error(PBS_ERR_ARG, "block_size must be 0, 64, 128 or 256");
   if (p->tile_tets != 0 && (p->tile_tets < 16 || p->tile_tets > 4096))
     return set_error(PBS_ERR_ARG, "tile_tets must be 0 or in [16, 4096]");
-  if (p->reserved != 0) return set_error(PBS_ERR_ARG, "reserved must be 0");
+  if (p->graph_passes < -1 || p->graph_passes > 1024) return set_error(PBS_ERR_ARG, "graph_passes must be in [-1, 1024]");
   return PBS_OK;
 }
 
@@ -136,7 +150,7 @@ static void make_devparams(pbs_solver *s) {
 
 // ---- kernel timing ----
 static int timed_begin(pbs_solver *s, int kid) {
-  if (!s->profiling) return -1;
+  if (!s->prof_period) return -1;
   if (s->pool_used + 2 > (int)s->pool.size()) {
     for (int i = 0; i < 64; ++i) {
       hipEvent_t e;
@@ -147,22 +161,33 @@ static int timed_begin(pbs_solver *s, int kid) {
   int i = s->pool_used;
   s->pool_used += 2;
   (void)hipEventRecord(s->pool[i], s->st);
-  s->pending.push_back({kid, i});
+  s->pending.push_back({kid, i, s->pass_ordinal});
   return i;
 }
 static void timed_end(pbs_solver *s, int i) {
   if (i >= 0) (void)hipEventRecord(s->pool[i + 1], s->st);
 }
-static void timed_collect(pbs_solver *s) {
+// Adds the recorded launch times to the per-kernel totals.  Descent-pass launches with an
+// ordinal >= executed ran after the loop had stopped (they return at once) and are not counted.
+static void timed_collect(pbs_solver *s, long executed) {
   for (auto &pr : s->pending) {
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, s->pool[pr.second], s->pool[pr.second + 1]) == hipSuccess) {
-      s->kt_ms[pr.first] += ms;
-      s->kt_n[pr.first] += 1;
+    if (pr.pass >= executed) continue;
+    if (hipEventElapsedTime(&ms, s->pool[pr.ev], s->pool[pr.ev + 1]) == hipSuccess) {
+      s->kt_ms[pr.kid] += ms;
+      s->kt_n[pr.kid] += 1;
     }
   }
   s->pending.clear();
   s->pool_used = 0;
+}
+
+static void graph_release(pbs_solver *s) {
+  for (auto &g : s->graph) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+    g.passes = 0;
+  }
 }
 
 // ---- state helpers ----
@@ -297,6 +322,9 @@ static void free_all(pbs_solver *s) {
   s->allocs.clear();
   for (auto e : s->pool) (void)hipEventDestroy(e);
   s->pool.clear();
+  graph_release(s);
+  for (auto e : s->gev) (void)hipEventDestroy(e);
+  s->gev.clear();
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->st) (void)hipStreamDestroy(s->st);
@@ -374,7 +402,7 @@ extern "C" int pbs_create(int nn, const float *nodes, const float *rest, const i
     if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device) != hipSuccess) maxlds = 65536;
     s->lds_budget = (size_t)maxlds;
   }
-  // Tile size (tets per tile): 512, halved while the mesh would give fewer than 1024 tiles, and
+  // Tile size (tets per tile): 256, halved while the mesh would give fewer than 1024 tiles, and
   // while the tile pass could need more than 40 KiB of LDS (four workgroups per 160 KiB CU); a
   // plan with tile_tets = t has at most t tets and max(64, t / 2) local nodes per tile.
   const bool f64 = p->precision == PBS_FP64;
@@ -386,7 +414,7 @@ extern "C" int pbs_create(int nn, const float *nodes, const float *rest, const i
   };
   int tt = p->tile_tets;
   if (tt == 0) {
-    tt = 512;
+    tt = 256;
     while (tt > 64 && (nt / tt < 1024 || lds_need(std::max(64, tt / 2), tt) > std::min<size_t>(s->lds_budget, 40960))) tt /= 2;
   }
   std::string msg;
@@ -502,16 +530,73 @@ extern "C" int pbs_get_velocities(pbs_solver *s, float *v, float *vl) {
 }
 
 // ---- the step ----
-template <typename R> static void launch_pass(pbs_solver *s) {
-  int i = timed_begin(s, K_TILE);
+template <typename R> static void launch_pass(pbs_solver *s, bool timed) {
+  int i = timed ? timed_begin(s, K_TILE) : -1;
   Launch<R>::tile_pass(mesh<R>(s), s->td, s->dp, s->st);
   timed_end(s, i);
-  i = timed_begin(s, K_NODE);
+  i = timed ? timed_begin(s, K_NODE) : -1;
   Launch<R>::node_pass(mesh<R>(s), s->td, s->dp, s->st);
   timed_end(s, i);
-  i = timed_begin(s, K_CONTROL);
+  i = timed ? timed_begin(s, K_CONTROL) : -1;
   Launch<R>::control(mesh<R>(s), s->td, s->dp, s->st);
   timed_end(s, i);
+}
+
+// Captures G descent passes into a HIP graph; the timed variant brackets each kernel with
+// external event-record nodes so that launch times are measured inside the replays.
+template <typename R> static int graph_build(pbs_solver *s, int G, int timed) {
+  auto &gs = s->graph[timed];
+  if (gs.exec) (void)hipGraphExecDestroy(gs.exec);
+  gs.exec = nullptr;
+  gs.passes = 0;
+  while (timed && s->gev.size() < 6 * (size_t)G) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    s->gev.push_back(e);
+  }
+  HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+  hipError_t err = hipSuccess;
+  for (int j = 0; j < G && err == hipSuccess; ++j)
+    for (int k = 0; k < 3 && err == hipSuccess; ++k) {
+      if (timed) err = hipEventRecordWithFlags(s->gev[6 * j + 2 * k], s->st, hipEventRecordExternal);
+      if (err != hipSuccess) break;
+      if (k == 0) Launch<R>::tile_pass(mesh<R>(s), s->td, s->dp, s->st);
+      else if (k == 1) Launch<R>::node_pass(mesh<R>(s), s->td, s->dp, s->st);
+      else Launch<R>::control(mesh<R>(s), s->td, s->dp, s->st);
+      err = hipGetLastError();
+      if (timed && err == hipSuccess) err = hipEventRecordWithFlags(s->gev[6 * j + 2 * k + 1], s->st, hipEventRecordExternal);
+    }
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(s->st, &g);
+  if (err == hipSuccess) err = e2;
+  if (err == hipSuccess) err = hipGraphInstantiate(&gs.exec, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  if (err != hipSuccess) {
+    gs.exec = nullptr;
+    return set_error(PBS_ERR_HIP, std::string("descent graph capture: ") + hipGetErrorString(err));
+  }
+  gs.dp = s->dp;
+  gs.passes = G;
+  return PBS_OK;
+}
+
+template <typename R> static int graph_ready(pbs_solver *s, int G, int timed) {
+  const auto &gs = s->graph[timed];
+  if (gs.exec && gs.passes == G && std::memcmp(&gs.dp, &s->dp, sizeof(DevParams)) == 0) return PBS_OK;
+  return graph_build<R>(s, G, timed);
+}
+
+// Adds the launch times of the first `executed` passes of the last timed replay.
+static void graph_collect(pbs_solver *s, long executed) {
+  static const int kid[3] = {K_TILE, K_NODE, K_CONTROL};
+  for (long j = 0; j < executed && j < s->graph[1].passes; ++j)
+    for (int k = 0; k < 3; ++k) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, s->gev[6 * j + 2 * k], s->gev[6 * j + 2 * k + 1]) == hipSuccess) {
+        s->kt_ms[kid[k]] += ms;
+        s->kt_n[kid[k]] += 1;
+      }
+    }
 }
 
 template <typename R> static int step_impl(pbs_solver *s, const float *ext, float *out, pbs_step_stats *stats) {
@@ -523,7 +608,17 @@ template <typename R> static int step_impl(pbs_solver *s, const float *ext, floa
   } else {
     s->dp.has_ext = 0;
   }
+  // descent passes go through captured graphs unless graph_passes < 0
+  const int G = s->p.graph_passes == 0 ? 32 : s->p.graph_passes;
+  const bool use_graph = G > 0;
+  const int period = s->prof_period;
+  if (use_graph) {
+    int rc = graph_ready<R>(s, G, 0);
+    if (!rc && period) rc = graph_ready<R>(s, G, 1);
+    if (rc) return rc;
+  }
   HIPCHK(hipEventRecord(s->ev0, s->st));
+  s->pass_ordinal = -1;
   int i = timed_begin(s, K_INIT);
   Launch<R>::step_init(m, s->dp, s->bs, s->st);
   timed_end(s, i);
@@ -531,18 +626,41 @@ template <typename R> static int step_impl(pbs_solver *s, const float *ext, floa
   const bool relg = s->dp.relgrad_tol > 0;
   const long cap = relg ? (long)s->p.max_iterations + 256 : (long)s->p.iterations + 256;
   long launched = 0;
-  int n = relg ? 32 : std::max(s->p.iterations, 1);
+  long n = relg ? 32 : std::max(s->p.iterations, 1);
   Ctrl c{};
   while (true) {
-    for (int k = 0; k < n; ++k) launch_pass<R>(s);
-    HIPCHK(hipGetLastError());
-    launched += n;
+    if (use_graph) {
+      const long reps = (n + G - 1) / G;
+      for (long r = 0; r < reps; ++r) {
+        const bool timed = period && (s->prof_tick++ % period) == 0;
+        if (timed) {  // the events are reused by the next timed replay: collect after this one
+          HIPCHK(hipMemcpyAsync(&c, m.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->st));
+          HIPCHK(hipStreamSynchronize(s->st));
+          const int before = c.passes;
+          HIPCHK(hipGraphLaunch(s->graph[1].exec, s->st));
+          HIPCHK(hipMemcpyAsync(&c, m.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->st));
+          HIPCHK(hipStreamSynchronize(s->st));
+          graph_collect(s, c.passes - before);
+        } else {
+          HIPCHK(hipGraphLaunch(s->graph[0].exec, s->st));
+        }
+        launched += G;
+      }
+    } else {
+      for (long k = 0; k < n; ++k) {
+        s->pass_ordinal = launched + k;
+        launch_pass<R>(s, period && (s->prof_tick++ % period) == 0);
+      }
+      HIPCHK(hipGetLastError());
+      launched += n;
+    }
     HIPCHK(hipMemcpyAsync(&c, m.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
     if (c.done) break;
     if (launched >= cap) return set_error(PBS_ERR_STATE, "descent loop did not terminate within the pass cap");
     n = relg ? 32 : std::max(1, c.budget - c.k) + 1;
   }
+  s->pass_ordinal = -1;
   i = timed_begin(s, K_FINAL);
   Launch<R>::step_final(m, s->dp, s->bs, s->st);
   timed_end(s, i);
@@ -553,7 +671,7 @@ template <typename R> static int step_impl(pbs_solver *s, const float *ext, floa
     HIPCHK(hipMemcpyAsync(out, s->stage, sizeof(float) * 3 * (size_t)s->nn, hipMemcpyDeviceToHost, s->st));
   }
   HIPCHK(hipStreamSynchronize(s->st));
-  if (s->profiling) timed_collect(s);
+  if (period) timed_collect(s, c.passes);
   if (stats) {
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
@@ -699,9 +817,11 @@ extern "C" int pbs_get_rest_data(pbs_solver *s, float *dminv, float *vol, float 
 }
 
 // ---- measurement ----
-extern "C" int pbs_set_profiling(pbs_solver *s, int on) {
+extern "C" int pbs_set_profiling(pbs_solver *s, int period) {
   if (!s) return set_error(PBS_ERR_ARG, "null handle");
-  s->profiling = on != 0;
+  if (period < 0) return set_error(PBS_ERR_ARG, "profiling period must be >= 0");
+  s->prof_period = period;
+  s->prof_tick = 0;
   return PBS_OK;
 }
 extern "C" int pbs_reset_kernel_times(pbs_solver *s) {

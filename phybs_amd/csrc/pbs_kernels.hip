@@ -269,7 +269,6 @@ k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
   const int b = xcd_tile(blockIdx.x, gridDim.x);
   const TileInfo ti = td.tiles[b];
   const int L = ti.own_cnt + ti.halo_cnt, N = p.nn, T = p.nt, tid = threadIdx.x;
-  const long H = td.halo_total;
   R *sx = (R *)smem;
   R *rec = sx + 4 * td.lcap;
   for (int i = tid; i < L; i += 256) {
@@ -334,13 +333,14 @@ k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
     const unsigned short *off = td.tile_off + ti.off_start;
     for (int i = tid; i < L; i += 256) {
       const int o0 = off[i], o1 = off[i + 1];
+      // owned partials: 3 planes of N; halo partials: 3 consecutive values per slot
       size_t d0, stride;
       if (i < ti.own_cnt) {
         d0 = (size_t)ti.own_start + i;
         stride = N;
       } else {
-        d0 = (size_t)ti.halo_off + (i - ti.own_cnt);
-        stride = H;
+        d0 = 3 * ((size_t)ti.halo_off + (i - ti.own_cnt));
+        stride = 1;
       }
       if (phase == 0) {
         R a0 = (R)0, a1 = (R)0, a2 = (R)0;
@@ -396,8 +396,12 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
   R *__restrict__ xn = ring.b[ctrl->nxt];
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   const int N = p.nn;
-  const long H = td.halo_total;
   double en = 0.0, g2 = 0.0;
+  if (ctrl->check_now) {  // this block's share of the tiles' elastic energies, in tile order
+    const int per = (td.ntiles + gridDim.x - 1) / gridDim.x, t0 = blockIdx.x * per;
+    const int t1 = min(td.ntiles, t0 + per);
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x) en += td.tpart[t];
+  }
   if (n < N) {
     R f[3];
     float hv[3];
@@ -405,17 +409,17 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
     for (int d = 0; d < 3; ++d) f[d] = fown[(size_t)d * N + n];
     const int b0 = td.bnd_off[n], b1 = td.bnd_off[n + 1];
     for (int e = b0; e < b1; ++e) {
-      const size_t slot = td.bnd[e];
+      const R *hp = hpart + 3 * (size_t)td.bnd[e];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) f[d] += hpart[d * H + slot];
+      for (int d = 0; d < 3; ++d) f[d] += hp[d];
     }
     if (hdnow) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) hv[d] = fownh[(size_t)d * N + n];
       for (int e = b0; e < b1; ++e) {
-        const size_t slot = td.bnd[e];
+        const float *hp = hparth + 3 * (size_t)td.bnd[e];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) hv[d] += hparth[d * H + slot];
+        for (int d = 0; d < 3; ++d) hv[d] += hp[d];
       }
 #pragma unroll
       for (int d = 0; d < 3; ++d) hd[(size_t)d * N + n] = hv[d];
@@ -448,7 +452,7 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
         }
         xn[i] = nx;
       }
-      en = (double)e;
+      en += (double)e;
     }
   }
   const double se = block_sum(en, lds_red);
@@ -459,23 +463,21 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
   }
 }
 
-// Control step: FP64 sums of the tile and node-block partials in a fixed order, then the loop
-// control of the reference.
-__global__ void __launch_bounds__(256) k_control(DevParams p, Ctrl *__restrict__ ctrl, TileDev td, int nnb) {
-  __shared__ double la[256], lc[256];
+// Control step: FP64 sums of the node blocks' partials (energy, |g|^2) in a fixed order, then the
+// loop control of the reference.
+__global__ void __launch_bounds__(1024) k_control(DevParams p, Ctrl *__restrict__ ctrl, TileDev td, int nnb) {
+  __shared__ double la[1024], lc[1024];
   if (ctrl->done) return;
-  const int tid = threadIdx.x, chk = ctrl->check_now;
+  const int tid = threadIdx.x;
   double a = 0.0, c = 0.0;
-  if (chk)
-    for (int i = tid; i < td.ntiles; i += 256) a += td.tpart[i];
-  for (int i = tid; i < nnb; i += 256) {
+  for (int i = tid; i < nnb; i += 1024) {
     a += td.npart[2 * i];
     c += td.npart[2 * i + 1];
   }
   la[tid] = a;
   lc[tid] = c;
   __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
+  for (int st = 512; st > 0; st >>= 1) {
     if (tid < st) {
       la[tid] += la[tid + st];
       lc[tid] += lc[tid + st];
@@ -667,7 +669,7 @@ void Launch<R>::node_pass(const DevMesh<R> &m, const TileDev &td, const DevParam
 }
 template <typename R>
 void Launch<R>::control(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
-  hipLaunchKernelGGL(k_control, dim3(1), dim3(256), 0, st, p, m.ctrl, td, nblk(p.nn, 256));
+  hipLaunchKernelGGL(k_control, dim3(1), dim3(1024), 0, st, p, m.ctrl, td, nblk(p.nn, 256));
 }
 template <typename R>
 void Launch<R>::step_init(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st) {

@@ -201,6 +201,36 @@ def test_bitwise_deterministic():
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
 
 
+@pytest.mark.parametrize("relgrad", [False, True])
+def test_graph_replay_matches_eager_launches(relgrad):
+    """Descent passes replayed from a captured HIP graph (8 passes per replay, so each step takes
+    several replays with host syncs and eager kernels between them) give bitwise the same
+    trajectory and loop statistics as launching the kernels one by one; with timing events
+    inside every replay, and inside every third replay, as well."""
+    rest, tets, mask, x0 = bar_case(twist=30.0)
+    say("graph replay vs eager, relgrad", relgrad)
+    kw = dict(precision=P.FP64, relgrad_tol=1e-4, max_iterations=5000) if relgrad else dict(alpha0=0.4)
+    res = []
+    for gp, prof in ((-1, 0), (8, 0), (8, 1), (8, 3)):
+        s = P.Solver(rest, tets, mask, P.default_params(graph_passes=gp, **kw))
+        s.set_positions(x0)
+        s.set_profiling(prof)
+        st = [s.step() for _ in range(4)]
+        ext = np.zeros_like(rest)
+        ext[-5:, 1] = 0.3
+        st.append(s.step(ext))
+        res.append((s.positions(f64=True), [(q.passes, q.iterations, q.rollbacks, q.g_norm) for q in st],
+                    s.kernel_times() if prof else None))
+        s.close()
+    for xg, stats, _ in res[1:]:
+        assert np.array_equal(xg.view(np.uint64), res[0][0].view(np.uint64))
+        assert stats == res[0][1]
+    kt = res[2][2]
+    passes = sum(p for p, _, _, _ in res[2][1])
+    assert kt["k_tile"][1] == passes and kt["k_node"][1] == passes and kt["k_control"][1] == passes
+    assert kt["k_tile"][0] > 0
+
+
 def test_rollback_path_matches_oracle():
     # alpha0 = 0.4 on the 30-degree bar overshoots once in step 3 (energy rises at a check)
     rest, tets, mask, x0 = bar_case(twist=30.0)
