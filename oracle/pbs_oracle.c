@@ -22,7 +22,8 @@
  *   orc_hessian_diag    diagonal of -df/dx by central differences of the forces
  *                       (test_oracle.py::test_hessian_diag_fd)
  *   orc_csr_pattern     brute force from the tet list: row pattern, diagonal index, kID
- *   orc_incidence       brute force (test_oracle.py::test_adjacency_*)
+ *                       (test_mesh.py::test_csr_pattern_bit_exact)
+ *   orc_incidence       brute force (test_mesh.py::test_incidence_bit_exact)
  *   orc_step            mass-only problem with the closed-form minimiser, REF_CPU double-counted
  *                       gravity minimiser, convergence to an independent scipy minimisation
  *                       of the incremental potential on small meshes, energy monotone on
@@ -33,7 +34,8 @@
  * tolerances used): the unconstrained SVD is one-sided Jacobi in FP64 (the reference calls
  * Eigen::JacobiSVD<Matrix3f> / McAdams), line-search energies are summed in FP64 (the
  * reference sums FP32 with atomics + thrust::reduce), per-node forces are gathered in
- * ascending tet order (the GPU reference uses float atomics).
+ * ascending tet order (the GPU reference uses float atomics; orc_params.gather_order = 1 sums in
+ * descending order, to size the FP32 order sensitivity, test_oracle.py::test_gather_order_*).
  */
 #include "pbs_oracle.h"
 
