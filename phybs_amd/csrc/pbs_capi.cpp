@@ -246,7 +246,7 @@ template <typename R> static int alloc_impl(pbs_solver *s) {
   const TilePlan &P = s->plan;
   const size_t H = std::max<long>(P.halo_total, 1);
   TileInfo *tiles;
-  int *halo, *bnd_off, *bnd;
+  int *halo, *bnd_off, *hslot;
   uint32_t *tlp;
   unsigned short *toff;
   rc |= dalloc(s, &tiles, (size_t)P.ntiles);
@@ -254,7 +254,7 @@ template <typename R> static int alloc_impl(pbs_solver *s) {
   rc |= dalloc(s, &tlp, 4 * T);
   rc |= dalloc(s, &toff, P.tile_off.size());
   rc |= dalloc(s, &bnd_off, N + 1);
-  rc |= dalloc(s, &bnd, std::max<size_t>(P.bnd.size(), 1));
+  rc |= dalloc(s, &hslot, H);
   rc |= dalloc(s, &s->td.tpart, (size_t)P.ntiles);
   rc |= dalloc(s, &s->td.npart, 2 * (N / 64 + 2));
   rc |= dalloc(s, &m.fown, 3 * N);
@@ -266,7 +266,7 @@ template <typename R> static int alloc_impl(pbs_solver *s) {
   s->td.tet_lp = (const uint4 *)tlp;
   s->td.tile_off = toff;
   s->td.bnd_off = bnd_off;
-  s->td.bnd = bnd;
+  s->td.halo_slot = hslot;
   rc |= dalloc(s, &m.ctrl, 1);
   rc |= dalloc(s, &m.flag, 1);
   rc |= dalloc(s, &m.tete, T);
@@ -298,8 +298,8 @@ template <typename R> static int setup_impl(pbs_solver *s, const int *mask) {
     HIPCHK(hipMemcpyAsync((void *)s->td.tet_lp, P.tet_lp.data(), sizeof(uint32_t) * P.tet_lp.size(), hipMemcpyHostToDevice, s->st));
   HIPCHK(hipMemcpyAsync((void *)s->td.tile_off, P.tile_off.data(), sizeof(uint16_t) * P.tile_off.size(), hipMemcpyHostToDevice, s->st));
   HIPCHK(hipMemcpyAsync((void *)s->td.bnd_off, P.bnd_off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, s->st));
-  if (!P.bnd.empty())
-    HIPCHK(hipMemcpyAsync((void *)s->td.bnd, P.bnd.data(), sizeof(int) * P.bnd.size(), hipMemcpyHostToDevice, s->st));
+  if (!P.halo_slot.empty())
+    HIPCHK(hipMemcpyAsync((void *)s->td.halo_slot, P.halo_slot.data(), sizeof(int) * P.halo_slot.size(), hipMemcpyHostToDevice, s->st));
   HIPCHK(hipMemsetAsync(m.fownh, 0, sizeof(float) * 3 * N, s->st));
   HIPCHK(hipMemsetAsync(m.flag, 0, sizeof(int), s->st));
   HIPCHK(hipMemsetAsync(m.hd, 0, sizeof(float) * 3 * N, s->st));

@@ -48,7 +48,8 @@ struct TileDev {
   const int *halo;
   const uint4 *tet_lp;          // per tet: corner local ids, corner contribution positions (16 bits each)
   const unsigned short *tile_off;
-  const int *bnd_off, *bnd;     // per node: halo partial slots of its force, in tile order
+  const int *bnd_off;           // per node: its halo partial slots [bnd_off[n], bnd_off[n+1]), tile order
+  const int *halo_slot;         // per halo entry: its partial slot
   double *tpart;                // per tile: elastic energy of its tets
   double *npart;                // per node block: node energy, |g|^2
 };

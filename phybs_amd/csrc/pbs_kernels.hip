@@ -339,7 +339,7 @@ k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
         d0 = (size_t)ti.own_start + i;
         stride = N;
       } else {
-        d0 = 3 * ((size_t)ti.halo_off + (i - ti.own_cnt));
+        d0 = 3 * (size_t)td.halo_slot[ti.halo_off + (i - ti.own_cnt)];
         stride = 1;
       }
       if (phase == 0) {
@@ -409,7 +409,7 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
     for (int d = 0; d < 3; ++d) f[d] = fown[(size_t)d * N + n];
     const int b0 = td.bnd_off[n], b1 = td.bnd_off[n + 1];
     for (int e = b0; e < b1; ++e) {
-      const R *hp = hpart + 3 * (size_t)td.bnd[e];
+      const R *hp = hpart + 3 * (size_t)e;
 #pragma unroll
       for (int d = 0; d < 3; ++d) f[d] += hp[d];
     }
@@ -417,7 +417,7 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
 #pragma unroll
       for (int d = 0; d < 3; ++d) hv[d] = fownh[(size_t)d * N + n];
       for (int e = b0; e < b1; ++e) {
-        const float *hp = hparth + 3 * (size_t)td.bnd[e];
+        const float *hp = hparth + 3 * (size_t)e;
 #pragma unroll
         for (int d = 0; d < 3; ++d) hv[d] += hp[d];
       }

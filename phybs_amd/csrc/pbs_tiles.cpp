@@ -201,9 +201,9 @@ bool build_tile_plan(int nn, const float *rest, int nt, const int *tets, int til
   P.halo_total = (long)P.halo.size();
   P.bnd_off.assign(nn + 1, 0);
   for (int n = 0; n < nn; ++n) P.bnd_off[n + 1] = P.bnd_off[n] + (int)bnd_lists[n].size();
-  P.bnd.reserve(P.bnd_off[nn]);
+  P.halo_slot.assign(P.halo.size(), 0);
   for (int n = 0; n < nn; ++n)
-    for (int v : bnd_lists[n]) P.bnd.push_back(v);
+    for (size_t j = 0; j < bnd_lists[n].size(); ++j) P.halo_slot[bnd_lists[n][j]] = P.bnd_off[n] + (int)j;
   return true;
 }
 

@@ -13,7 +13,8 @@ namespace pbs {
 // Tiles are cut so that each holds at most tile_tets tets, 1024 owned nodes and, unless one
 // node's tets alone exceed it, max(64, tile_tets / 2) local nodes.  A tile's tets also reach "halo"
 // nodes owned by later tiles; the tile writes one partial force per owned and per halo node,
-// and the node pass adds a node's partials in tile order.
+// and the node pass adds a node's partials in tile order.  Halo partial slots are numbered by
+// owner node, so each node's partials are contiguous and the node pass reads them coalesced.
 //
 // Inside a tile, node i (local id: owned nodes first, then halo nodes) receives the corner
 // contributions run [off[i], off[i+1]) of the tile's contribution array, ordered by tet, then
@@ -42,7 +43,8 @@ struct TilePlan {
   std::vector<uint16_t> tile_off;   // per tile: local node count + 1 run offsets
   std::vector<uint32_t> tet_lp;     // 4 words per tet: local ids (c0 | c1 << 16, c2 | c3 << 16),
                                     // contribution positions (likewise)
-  std::vector<int> bnd_off, bnd;    // per node: halo partial slots holding part of its force, tile order
+  std::vector<int> bnd_off;         // per node: its halo partial slots [bnd_off[n], bnd_off[n+1]), tile order
+  std::vector<int> halo_slot;       // per halo entry (tile, halo node): its partial slot
 };
 
 // Builds the plan.  rest: 3N AoS; tets: 4T.  tile_tets: max tets per tile (1 .. 16383).

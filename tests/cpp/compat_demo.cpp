@@ -57,7 +57,7 @@ int main(int argc, char **argv) {
   std::vector<float> rest(3 * nn);
   std::vector<int> tets(4 * nt);
   phybs::check(pbs_mesh_bar(nx, ny, nz, edge, rest.data(), tets.data()));
-  std::vector<float> x0(3 * nn);
+  std::vector<float> x0(rest);  // twisted in place
   phybs::check(pbs_mesh_twist(nx, ny, nz, edge, 30.0, 0.0, 1, nn, x0.data()));
   std::vector<int> mask(nn);
   for (int i = 0; i < nn; ++i) mask[i] = rest[3 * i] == 0.0f;
