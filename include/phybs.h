@@ -85,6 +85,7 @@ typedef struct pbs_step_stats {
   double g0_norm;   /* |g| at the plain initial guess, free DOFs */
   double g_norm;    /* |g| at the last evaluated iterate */
   double gpu_ms;    /* device time of the step (HIP events on the handle's stream) */
+  double g_learned; /* |g| at the learned initial guess (0 when the warm start is off) */
 } pbs_step_stats;
 
 /* ---- parameters / errors ---- */
@@ -179,6 +180,49 @@ int64_t pbs_csr_pattern(int num_nodes, int num_tets, const int *tets, int *rowpt
 /* Tet incidence per node in ascending tet order (ProjDynamic::initLocalProjection,
  * ProjDynamic.cpp:25-45): offsets N+1, entries 4T = 12*tet + 3*local. */
 int pbs_incidence(int num_nodes, int num_tets, const int *tets, int *offsets, int *entries);
+
+/* ---- learned warm start (no counterpart in the reference, which has no learned component) ----
+ * A per-vertex network (18 features -> 64 -> 64 -> 64 -> 3, SiLU; BF16 weights on the matrix
+ * cores, FP32 accumulation) predicts the displacement of the converged step from the inertial
+ * target y = x + h v + h^2 g.  Features per free node at the step start: u = x - X, h v,
+ * s = h v + h^2 g and the 1-ring means of (u, h v, s) minus the node's own.  With the warm start
+ * enabled, pbs_step evaluates the gradient at the plain and at the learned guess and starts from
+ * the learned one only if its gradient norm is not larger; the plain norm remains g0_norm (the
+ * rel-grad denominator).  Parameters: PBS_MLP_PARAMS floats, packed W1 (18x64, in x out), b1,
+ * W2 (64x64), b2, W3, b3, W4 (64x3), b4; PBS_MLP_SCALES scales (6 feature groups, output). */
+#define PBS_MLP_PARAMS 9731
+#define PBS_MLP_SCALES 7
+int pbs_learn_set_model(pbs_solver *s, const float *params, const float *scales);
+int pbs_learn_get_model(pbs_solver *s, float *params, float *scales);
+/* Glorot-uniform weights from seed, zero biases (scales unchanged). */
+int pbs_learn_init_model(pbs_solver *s, uint64_t seed);
+int pbs_learn_enable(pbs_solver *s, int on);
+/* How the learned guess is used: the correction d = learned - plain is smoothed by `sweeps`
+ * averaging sweeps over the node graph (d <- d/2 + (1-ring mean of d)/2, fixed nodes at 0), then
+ * the candidates plain + beta_j d (j < nbeta, 1..16 steps >= 0) are each evaluated with one
+ * gradient pass and the best is used if its gradient norm is not above the plain guess's.
+ * Defaults: 0 sweeps, one step of 1. */
+int pbs_learn_set_options(pbs_solver *s, int sweeps, int nbeta, const float *betas);
+/* Starts collecting (features, converged x - y) of every free node from each pbs_step, up to
+ * capacity samples (0 stops and frees); collection stops when the next step would not fit.
+ * Samples are stored step after step, the free nodes of a step in the caller's node order. */
+int pbs_learn_collect(pbs_solver *s, int64_t capacity);
+int64_t pbs_learn_samples(pbs_solver *s);
+int pbs_learn_get_samples(pbs_solver *s, int64_t start, int64_t n, float *features, float *targets);
+/* Scales = RMS of each feature group and of the targets over the collected samples. */
+int pbs_learn_fit_scales(pbs_solver *s);
+/* Adam (lr, beta 0.9/0.999) on the mean squared error of the normalised outputs, mini-batches of
+ * `batch` shuffled samples, on the GPU; loss_per_epoch (may be NULL) gets each epoch's mean loss. */
+int pbs_learn_train(pbs_solver *s, int epochs, int batch, float lr, uint64_t seed, float *loss_per_epoch);
+/* Network outputs (normalised, 3 per sample) for n feature rows (18 each). */
+int pbs_learn_forward(pbs_solver *s, int n, const float *features, float *out);
+/* Mean loss and gradient (PBS_MLP_PARAMS) over n samples, without updating the model. */
+int pbs_learn_loss_grad(pbs_solver *s, int n, const float *features, const float *targets, double *loss, float *grad);
+/* Features of the current state (N x 18, caller's node order; zero rows for fixed nodes). */
+int pbs_learn_features(pbs_solver *s, float *features);
+/* The learned and the plain initial guess of the next step (3N AoS doubles each, either may be
+ * NULL); the solver state is not changed. */
+int pbs_learn_guess(pbs_solver *s, double *learned, double *plain);
 
 #ifdef __cplusplus
 }

@@ -565,7 +565,8 @@ static int FN(step)(orc_solver *s, const double *extd, const double *x0o, orc_st
   out->passes = passes;
   out->iterations = k;
   out->rollbacks = rollbacks;
-  out->converged = converged | (used_learned ? 2 : 0);
+  out->converged = converged;
+  out->used_learned = used_learned;
   out->alpha = alpha;
   out->energy = Eold;
   out->g0_norm = g0;

@@ -65,6 +65,7 @@ typedef struct orc_stats {
   double energy;        /* last checked energy */
   double g0_norm;       /* |g(x_0)| over free DOFs */
   double g_norm;        /* |g| of the last evaluated iterate */
+  int used_learned;     /* the step started from the caller's guess (its gradient was not larger) */
 } orc_stats;
 
 /* One record per loop pass (for tests of the control flow). */

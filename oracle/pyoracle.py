@@ -31,6 +31,7 @@ class Stats(C.Structure):
     _fields_ = [
         ("passes", C.c_int), ("iterations", C.c_int), ("rollbacks", C.c_int), ("converged", C.c_int),
         ("alpha", C.c_float), ("energy", C.c_double), ("g0_norm", C.c_double), ("g_norm", C.c_double),
+        ("used_learned", C.c_int),
     ]
 
 

@@ -42,7 +42,7 @@ class StepStats(C.Structure):
     _fields_ = [
         ("passes", C.c_int), ("iterations", C.c_int), ("rollbacks", C.c_int), ("converged", C.c_int),
         ("used_learned", C.c_int), ("alpha", C.c_float), ("energy", C.c_double), ("g0_norm", C.c_double),
-        ("g_norm", C.c_double), ("gpu_ms", C.c_double),
+        ("g_norm", C.c_double), ("gpu_ms", C.c_double), ("g_learned", C.c_double),
     ]
 
     def as_dict(self):
@@ -83,7 +83,22 @@ SIGNATURES = {
     "pbs_node_read": (C.c_int, [C.c_char_p, C.c_int, _P]),
     "pbs_csr_pattern": (C.c_int64, [C.c_int, C.c_int, _P, _P, _P, _P, _P]),
     "pbs_incidence": (C.c_int, [C.c_int, C.c_int, _P, _P, _P]),
+    "pbs_learn_set_model": (C.c_int, [_P, _P, _P]),
+    "pbs_learn_get_model": (C.c_int, [_P, _P, _P]),
+    "pbs_learn_init_model": (C.c_int, [_P, C.c_uint64]),
+    "pbs_learn_enable": (C.c_int, [_P, C.c_int]),
+    "pbs_learn_set_options": (C.c_int, [_P, C.c_int, C.c_int, _P]),
+    "pbs_learn_collect": (C.c_int, [_P, C.c_int64]),
+    "pbs_learn_samples": (C.c_int64, [_P]),
+    "pbs_learn_get_samples": (C.c_int, [_P, C.c_int64, C.c_int64, _P, _P]),
+    "pbs_learn_fit_scales": (C.c_int, [_P]),
+    "pbs_learn_train": (C.c_int, [_P, C.c_int, C.c_int, C.c_float, C.c_uint64, _P]),
+    "pbs_learn_forward": (C.c_int, [_P, C.c_int, _P, _P]),
+    "pbs_learn_loss_grad": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),
+    "pbs_learn_features": (C.c_int, [_P, _P]),
+    "pbs_learn_guess": (C.c_int, [_P, _P, _P]),
 }
+MLP_PARAMS, MLP_SCALES = 9731, 7
 
 _lib = None
 
@@ -283,6 +298,78 @@ class Solver:
 
     def reset_kernel_times(self):
         _check(lib().pbs_reset_kernel_times(self.h))
+
+    # ---- learned warm start ----
+    def learn_set_model(self, params, scales):
+        p = np.ascontiguousarray(params, np.float32)
+        sc = np.ascontiguousarray(scales, np.float32)
+        assert p.size == MLP_PARAMS and sc.size == MLP_SCALES
+        _check(lib().pbs_learn_set_model(self.h, _p(p), _p(sc)))
+
+    def learn_get_model(self):
+        p = np.zeros(MLP_PARAMS, np.float32)
+        sc = np.zeros(MLP_SCALES, np.float32)
+        _check(lib().pbs_learn_get_model(self.h, _p(p), _p(sc)))
+        return p, sc
+
+    def learn_init_model(self, seed=0):
+        _check(lib().pbs_learn_init_model(self.h, seed))
+
+    def learn_enable(self, on=True):
+        _check(lib().pbs_learn_enable(self.h, 1 if on else 0))
+
+    def learn_set_options(self, sweeps=0, betas=(1.0,)):
+        b = np.ascontiguousarray(betas, np.float32)
+        _check(lib().pbs_learn_set_options(self.h, int(sweeps), len(b), _p(b)))
+
+    def learn_collect(self, capacity):
+        _check(lib().pbs_learn_collect(self.h, int(capacity)))
+
+    def learn_samples(self):
+        n = lib().pbs_learn_samples(self.h)
+        if n < 0:
+            _check(int(n))
+        return int(n)
+
+    def learn_get_samples(self, start=0, n=None):
+        n = self.learn_samples() - start if n is None else n
+        f = np.zeros((n, 18), np.float32)
+        t = np.zeros((n, 3), np.float32)
+        _check(lib().pbs_learn_get_samples(self.h, start, n, _p(f), _p(t)))
+        return f, t
+
+    def learn_fit_scales(self):
+        _check(lib().pbs_learn_fit_scales(self.h))
+
+    def learn_train(self, epochs, batch=4096, lr=1e-3, seed=0):
+        loss = np.zeros(max(epochs, 1), np.float32)
+        _check(lib().pbs_learn_train(self.h, epochs, batch, lr, seed, _p(loss)))
+        return loss[:epochs]
+
+    def learn_forward(self, features):
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, 18)
+        out = np.zeros((len(f), 3), np.float32)
+        _check(lib().pbs_learn_forward(self.h, len(f), _p(f), _p(out)))
+        return out
+
+    def learn_loss_grad(self, features, targets):
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, 18)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        loss = C.c_double()
+        g = np.zeros(MLP_PARAMS, np.float32)
+        _check(lib().pbs_learn_loss_grad(self.h, len(f), _p(f), _p(t), C.byref(loss), _p(g)))
+        return loss.value, g
+
+    def learn_features(self):
+        out = np.zeros((self.nn, 18), np.float32)
+        _check(lib().pbs_learn_features(self.h, _p(out)))
+        return out
+
+    def learn_guess(self):
+        a = np.zeros((self.nn, 3), np.float64)
+        b = np.zeros((self.nn, 3), np.float64)
+        _check(lib().pbs_learn_guess(self.h, _p(a), _p(b)))
+        return a, b
 
     def kernel_times(self):
         n = 16

@@ -16,6 +16,8 @@
 
 #include "pbs_host.h"
 #include "pbs_internal.h"
+#include "pbs_learn.h"
+#include "pbs_solver.h"
 #include "pbs_tiles.h"
 #include "phybs.h"
 
@@ -28,72 +30,6 @@ int set_error(int code, const std::string &msg) {
 }  // namespace pbs
 
 using namespace pbs;
-
-#define HIPCHK(expr)                                                                                  \
-  do {                                                                                                \
-    hipError_t e_ = (expr);                                                                           \
-    if (e_ != hipSuccess) return set_error(PBS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-namespace {
-enum KernelId { K_TILE = 0, K_NODE, K_CONTROL, K_INIT, K_FINAL, K_COUNT };
-const char *kKernelNames[K_COUNT] = {"k_tile", "k_node", "k_control", "k_step_init", "k_step_final"};
-}  // namespace
-
-struct pbs_solver {
-  int nn = 0, nt = 0;
-  pbs_params p{};
-  DevParams dp{};
-  int bs = 256, ntb = 0, nnb = 0;
-  hipStream_t st = nullptr;
-  int device = 0;
-  DevMesh<float> m32{};
-  DevMesh<double> m64{};
-  TileDev td{};
-  TilePlan plan;
-  size_t lds_budget = 65536;
-  std::vector<float> rest_h;
-  float *stage = nullptr;     // 3N float AoS staging
-  double *stage64 = nullptr;  // 3N double AoS staging
-  std::vector<void *> allocs;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  std::vector<hipEvent_t> pool;
-  struct Pending {
-    int kid, ev;  // kernel id, first of its two events
-    long pass;    // pass ordinal within the step (-1: not a descent pass)
-  };
-  std::vector<Pending> pending;
-  int pool_used = 0;
-  long pass_ordinal = -1;  // ordinal given to the descent-pass launches being recorded
-  // HIP graphs of graph_passes descent passes, replayed by pbs_step: [0] plain, [1] with
-  // timing events around each kernel (6 per pass, gev)
-  struct PassGraph {
-    hipGraphExec_t exec = nullptr;
-    DevParams dp{};  // parameters baked into the graph
-    int passes = 0;
-  } graph[2];
-  std::vector<hipEvent_t> gev;
-  int prof_period = 0;  // 0: no timing; n: time every n-th graph replay (or pass without graphs)
-  long prof_tick = 0;
-  double kt_ms[K_COUNT] = {0};
-  long kt_n[K_COUNT] = {0};
-};
-
-template <typename R> static DevMesh<R> &mesh(pbs_solver *s);
-template <> DevMesh<float> &mesh<float>(pbs_solver *s) { return s->m32; }
-template <> DevMesh<double> &mesh<double>(pbs_solver *s) { return s->m64; }
-
-template <typename T> static int dalloc(pbs_solver *s, T **p, size_t n) {
-  void *q = nullptr;
-  hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
-  if (e != hipSuccess) {
-    *p = nullptr;
-    return set_error(PBS_ERR_HIP, std::string("hipMalloc(") + std::to_string(n * sizeof(T)) + " B): " + hipGetErrorString(e));
-  }
-  s->allocs.push_back(q);
-  *p = (T *)q;
-  return PBS_OK;
-}
 
 static int validate_params(const pbs_params *p) {
   if (!p) return set_error(PBS_ERR_ARG, "null params");
@@ -323,6 +259,8 @@ static void free_all(pbs_solver *s) {
   for (auto e : s->pool) (void)hipEventDestroy(e);
   s->pool.clear();
   graph_release(s);
+  learn_release(s->learn);
+  s->learn = nullptr;
   for (auto e : s->gev) (void)hipEventDestroy(e);
   s->gev.clear();
   if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -623,6 +561,12 @@ template <typename R> static int step_impl(pbs_solver *s, const float *ext, floa
   Launch<R>::step_init(m, s->dp, s->bs, s->st);
   timed_end(s, i);
   HIPCHK(hipGetLastError());
+  if (s->learn) {  // features / learned guess / selection by gradient norm
+    i = timed_begin(s, K_LEARN);
+    int rc = learn_before_descent<R>(s);
+    timed_end(s, i);
+    if (rc) return rc;
+  }
   const bool relg = s->dp.relgrad_tol > 0;
   const long cap = relg ? (long)s->p.max_iterations + 256 : (long)s->p.iterations + 256;
   long launched = 0;
@@ -665,6 +609,10 @@ template <typename R> static int step_impl(pbs_solver *s, const float *ext, floa
   Launch<R>::step_final(m, s->dp, s->bs, s->st);
   timed_end(s, i);
   HIPCHK(hipGetLastError());
+  if (s->learn) {
+    int rc = learn_after_step<R>(s);
+    if (rc) return rc;
+  }
   HIPCHK(hipEventRecord(s->ev1, s->st));
   if (out) {
     Launch<R>::planes_to_aos(m.xpos, s->stage, m.perm, s->nn, s->st);
@@ -685,6 +633,7 @@ template <typename R> static int step_impl(pbs_solver *s, const float *ext, floa
     stats->g0_norm = c.g0;
     stats->g_norm = c.g;
     stats->gpu_ms = ms;
+    stats->g_learned = c.used_learned || (s->learn && s->learn->enabled) ? std::sqrt(c.g_learned) : 0.0;
   }
   return PBS_OK;
 }

@@ -36,8 +36,9 @@ struct Ctrl {
   int rollbacks, converged, used_learned, first;
   int cur, nxt, last, old;  // roles of the 4 position buffers
   int hd_now, check_now, cheb_now, kc;  // kc: iteration of the last Chebyshev restart
-  float alpha, rho, omega, rho_next, omega_next, pad2;
+  float alpha, rho, omega, rho_next, omega_next, ws_beta;  // ws_beta: warm start's chosen step
   double E, E_old, g0, g;
+  double g_plain, g_learned;  // warm start: gradient norms of the two initial guesses
 };
 
 // Tile partition on the device (see pbs_tiles.h for the host-side construction).
@@ -100,6 +101,7 @@ template <typename R> struct Launch {
   static size_t tile_lds(const TileDev &td);
   static void tile_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
   static void node_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
+  static void grad_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
   static void control(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
   static void step_init(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st);
   static void step_final(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st);
@@ -109,6 +111,42 @@ template <typename R> struct Launch {
   static void planes_to_aos(const R *planes, float *aos, const int *perm, int n, hipStream_t st);
   static void planes_to_aos64(const R *planes, double *aos, const int *perm, int n, hipStream_t st);
 };
+// ---- learned warm start (pbs_learn.hip) ----
+// Per-vertex network 18 -> 64 -> 64 -> 64 -> 3, SiLU; packed FP32 parameters W1 (18x64, in x out),
+// b1, W2 (64x64), b2, W3, b3, W4 (64x3), b4.
+constexpr int MLP_NPARAM = 18 * 64 + 64 + 2 * (64 * 64 + 64) + 64 * 3 + 3;  // 9731
+constexpr int MLP_NSCALE = 7;       // 6 feature-group scales, 1 output scale
+constexpr int MLP_WA = 11264;       // BF16 weights arranged as MFMA A operands (all 4 layers)
+struct MlpDev {
+  const float *params;         // packed FP32 master parameters
+  const unsigned short *wA;    // BF16 A-operand arrangement of the weights
+  float inv_s[MLP_NSCALE];     // float32(1) / scale
+};
+template <typename R> struct LearnLaunch {
+  // features of every node at the current step start (x = xpos, v) into feat (N x 18, FP32), and
+  // the inertial target y = x + h v + h^2 g (planes) into y
+  static void features(const DevMesh<R> &m, const DevParams &p, const int *adj_off, const int *adj, float *feat,
+                       R *y, hipStream_t st);
+  // learned guess y + out * scale into ring.b[1] for free nodes (fixed nodes copy ring.b[0])
+  static void guess(const DevMesh<R> &m, const DevParams &p, const MlpDev &net, const float *feat, const R *y,
+                    hipStream_t st);
+  // correction d = ring.b[1] - ring.b[0], smoothed by iters sweeps (ring.b[2..3] hold d and
+  // scratch), then the selection: the gradient norm of the plain guess ring.b[0] and of
+  // ring.b[0] + beta d for each beta; the descent starts from the best of them
+  static void select(const DevMesh<R> &m, const TileDev &td, const DevParams &p, const int *adj_off, const int *adj,
+                     int iters, const float *betas, int nbeta, hipStream_t st);
+  // appends (features, x_new - y) of free nodes to the sample buffers
+  static void collect(const DevMesh<R> &m, const DevParams &p, const float *feat, const R *y, const int *free_rank,
+                      long base, float *ds_x, float *ds_y, hipStream_t st);
+};
+void mlp_arrange(const float *params, unsigned short *wA, hipStream_t st);
+void mlp_forward(const MlpDev &net, const float *feat, int n, float *out, hipStream_t st);
+// training: per 32-sample workgroup partial gradients (MLP_NPARAM each) and loss sums
+void mlp_train_partials(const MlpDev &net, const float *ds_x, const float *ds_y, const int *idx, int n, int batch,
+                        float *grad_part, float *loss_part, hipStream_t st);
+void mlp_reduce(const float *grad_part, const float *loss_part, int nwg, float *grad, double *loss_acc, hipStream_t st);
+void mlp_adam(float *params, float *m1, float *m2, const float *grad, float lr, float b1t, float b2t, hipStream_t st);
+
 void aos_to_planes_f(const float *aos, float *planes, const int *perm, int n, hipStream_t st);
 void planes_to_aos_f(const float *planes, float *aos, const int *perm, int n, hipStream_t st);
 void fill_f(float *p, float v, long n, hipStream_t st);

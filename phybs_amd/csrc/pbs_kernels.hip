@@ -381,7 +381,8 @@ k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
 // partial + the node's halo partials in tile order; Jacobi-preconditioned step with Chebyshev.
 // g = (x - x~) m/h^2 - f - s (m grav + f_ext), s = 1 (REF_GPU) or 2 (REF_CPU, DescentOptimize.cpp:51-52).
 // ---------------------------------------------------------------------------------------------
-template <typename R>
+// GRAD_ONLY: only the |g|^2 partials (the warm start's gradient passes); nothing else is written.
+template <typename R, bool GRAD_ONLY>
 __global__ void __launch_bounds__(256)
 k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, const R *__restrict__ fown,
        const R *__restrict__ hpart, const float *__restrict__ fownh, const float *__restrict__ hparth,
@@ -397,7 +398,7 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   const int N = p.nn;
   double en = 0.0, g2 = 0.0;
-  if (ctrl->check_now) {  // this block's share of the tiles' elastic energies, in tile order
+  if (!GRAD_ONLY && ctrl->check_now) {  // this block's share of the tiles' elastic energies, in tile order
     const int per = (td.ntiles + gridDim.x - 1) / gridDim.x, t0 = blockIdx.x * per;
     const int t1 = min(td.ntiles, t0 + per);
     for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x) en += td.tpart[t];
@@ -413,7 +414,10 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
 #pragma unroll
       for (int d = 0; d < 3; ++d) f[d] += hp[d];
     }
-    if (hdnow) {
+    if (GRAD_ONLY) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) hv[d] = 0.0f;
+    } else if (hdnow) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) hv[d] = fownh[(size_t)d * N + n];
       for (int e = b0; e < b1; ++e) {
@@ -428,8 +432,9 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
       for (int d = 0; d < 3; ++d) hv[d] = hd[(size_t)d * N + n];
     }
     if (mask[n]) {
+      if (!GRAD_ONLY)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) xn[(size_t)d * N + n] = x[(size_t)d * N + n];
+        for (int d = 0; d < 3; ++d) xn[(size_t)d * N + n] = x[(size_t)d * N + n];
     } else {
       const float m = mass[n];
       const R hstep = (R)p.h;
@@ -445,14 +450,16 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
         const R gr = dd * mh2 - f[d] - (R)p.ext_scale * fe;
         e += (R)0.5 * dd * dd * mh2 - fe * xc;
         g2 += (double)gr * (double)gr;
-        R nx = xc - alpha * gr / ((R)hv[d] + mh2);
-        if (cheb) {
-          const R l = xl[i];
-          nx = om * (nx - l) + l;
+        if (!GRAD_ONLY) {
+          R nx = xc - alpha * gr / ((R)hv[d] + mh2);
+          if (cheb) {
+            const R l = xl[i];
+            nx = om * (nx - l) + l;
+          }
+          xn[i] = nx;
         }
-        xn[i] = nx;
       }
-      en += (double)e;
+      if (!GRAD_ONLY) en += (double)e;
     }
   }
   const double se = block_sum(en, lds_red);
@@ -537,11 +544,13 @@ __global__ void k_step_init(DevParams p, Ctrl *__restrict__ ctrl, Ring<R> ring, 
     C.rho_next = rho;
     C.omega_next = om;
     C.cheb_now = om != 1.0f;
-    C.pad2 = 0.0f;
+    C.ws_beta = 0.0f;
     C.E = 0.0;
     C.E_old = 0.0;
     C.g0 = -1.0;
     C.g = 0.0;
+    C.g_plain = 0.0;
+    C.g_learned = 0.0;
     C.done = (p.iterations <= 0 || p.alpha0 < p.alpha_min) ? 1 : 0;
     *ctrl = C;
   }
@@ -664,7 +673,12 @@ void Launch<R>::tile_pass(const DevMesh<R> &m, const TileDev &td, const DevParam
 }
 template <typename R>
 void Launch<R>::node_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
-  hipLaunchKernelGGL((k_node<R>), dim3(nblk(p.nn, 256)), dim3(256), 0, st, p, m.ctrl, m.ring, td, m.fown, m.hpart,
+  hipLaunchKernelGGL((k_node<R, false>), dim3(nblk(p.nn, 256)), dim3(256), 0, st, p, m.ctrl, m.ring, td, m.fown, m.hpart,
+                     m.fownh, m.hparth, m.hd, m.mass, m.mask, m.ext, m.xt);
+}
+template <typename R>
+void Launch<R>::grad_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
+  hipLaunchKernelGGL((k_node<R, true>), dim3(nblk(p.nn, 256)), dim3(256), 0, st, p, m.ctrl, m.ring, td, m.fown, m.hpart,
                      m.fownh, m.hparth, m.hd, m.mass, m.mask, m.ext, m.xt);
 }
 template <typename R>

@@ -1,0 +1,252 @@
+"""GPU tests of the learned warm start: the MFMA network and the training kernels against the numpy
+specification (oracle/mlp_oracle.py), the features, the fallback, the warm-started step against
+the CPU oracle's warm-started step, and what a trained network gains."""
+import sys
+
+import numpy as np
+import pytest
+
+import meshgen as M
+import phybs_amd as P
+from oracle import mlp_oracle as MO
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+SCALES = np.array([0.5, 1.0, 2.0, 0.3, 0.7, 1.5, 0.8], np.float32)
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def bar(twist=30.0, jitter=0.01):
+    rest, tets = M.bar(4, 4, 16, 0.05)
+    mask = M.clamp_x0(rest)
+    x0 = M.twisted_state(rest, 4, 4, 16, 0.05, twist, jitter, 1)
+    return rest, tets, mask, x0
+
+
+def model(seed=3):
+    p = MO.init_params(seed)
+    p[1152:1216] = 0.1  # nonzero first-layer biases
+    return p
+
+
+def test_mlp_forward_matches_spec():
+    rest, tets, mask, _ = bar()
+    say("MFMA forward vs numpy")
+    s = P.Solver(rest, tets, mask, P.default_params())
+    s.learn_set_model(model(), SCALES)
+    f = np.random.default_rng(0).normal(0, 1, (1000, 18)).astype(np.float32)
+    got = s.learn_forward(f)
+    ref, _ = MO.forward(model(), SCALES, f)
+    err = np.abs(got - ref)
+    scale = np.abs(ref).max()
+    say("max err", err.max(), "of", scale, "fraction above 1e-4 x max:", np.mean(err > 1e-4 * scale))
+    # FP32 accumulation order differs; a hidden activation whose BF16 rounding flips moves an
+    # output by ~1e-3 of its scale, which happens for a small fraction of samples
+    assert err.max() <= 1e-2 * scale
+    assert np.mean(err > 1e-4 * scale) <= 0.01
+    s.close()
+
+
+def test_mlp_loss_and_gradient_match_spec():
+    rest, tets, mask, _ = bar()
+    say("training gradient vs numpy")
+    s = P.Solver(rest, tets, mask, P.default_params())
+    s.learn_set_model(model(), SCALES)
+    rng = np.random.default_rng(1)
+    f = rng.normal(0, 1, (777, 18)).astype(np.float32)
+    t = rng.normal(0, 1, (777, 3)).astype(np.float32)
+    loss, g = s.learn_loss_grad(f, t)
+    lref, gref = MO.loss_grad(model(), SCALES, f, t)
+    say("loss", loss, lref, "grad rel err", np.linalg.norm(g - gref) / np.linalg.norm(gref))
+    assert loss == pytest.approx(lref, rel=1e-5)
+    assert np.linalg.norm(g - gref) <= 1e-3 * np.linalg.norm(gref)
+    s.close()
+
+
+def adjacency(nn, tets):
+    nb = [set() for _ in range(nn)]
+    for t in tets:
+        for a in t:
+            for b in t:
+                if a != b:
+                    nb[a].add(int(b))
+    off = np.zeros(nn + 1, np.int64)
+    adj = []
+    for n in range(nn):
+        adj += sorted(nb[n])
+        off[n + 1] = len(adj)
+    return off, np.array(adj, np.int64)
+
+
+def test_features_match_spec():
+    rest, tets, mask, x0 = bar()
+    say("features")
+    h = float(np.float32(0.03))
+    s = P.Solver(rest, tets, mask, P.default_params(precision=P.FP64))
+    s.set_positions(x0)
+    for _ in range(2):
+        s.step()
+    got = s.learn_features()
+    x = s.positions(f64=True)
+    v = s.velocities()[0]
+    off, adj = adjacency(len(rest), tets)
+    ref = MO.features(x, v, rest, h, np.float32([0.0, -9.8, 0.0]).astype(np.float64), off, adj, mask)
+    scale = np.abs(ref).max(axis=0) + 1e-30
+    err = (np.abs(got - ref) / scale).max()
+    say("max rel err (per feature scale)", err)
+    assert err <= 1e-5  # velocities cross the boundary as FP32
+    assert not got[mask != 0].any()
+    s.close()
+
+
+def test_rejected_guess_leaves_the_plain_trajectory_unchanged():
+    rest, tets, mask, x0 = bar()
+    say("fallback: a far-off guess is rejected, the trajectory is bitwise the plain one")
+    bad_scales = SCALES.copy()
+    bad_scales[6] = 1e3  # outputs of order 1 become displacements of order 1 km
+    runs = []
+    for warm in (False, True):
+        s = P.Solver(rest, tets, mask, P.default_params(precision=P.FP64, relgrad_tol=1e-4, max_iterations=5000))
+        s.set_positions(x0)
+        if warm:
+            s.learn_set_model(model(), bad_scales)
+            s.learn_enable(True)
+        st = [s.step() for _ in range(3)]
+        runs.append((s.positions(f64=True), st))
+        s.close()
+    (xp, sp), (xl, sl) = runs
+    # the far-off guess has a larger (or non-finite) gradient norm and is not used
+    assert all(q.used_learned == 0 and not (q.g_learned <= q.g0_norm) for q in sl)
+    assert [q.iterations for q in sp] == [q.iterations for q in sl]
+    assert [q.g0_norm for q in sp] == [q.g0_norm for q in sl]
+    assert np.array_equal(xp.view(np.uint64), xl.view(np.uint64))
+
+
+@pytest.mark.parametrize("net", ["zero", "random"])
+def test_warm_started_step_matches_oracle(net):
+    """FP64: the GPU's learned guess handed to the oracle's warm-started step (which evaluates both
+    guesses' gradients and keeps the better) gives the same decision, iterations and positions.
+    The zero network guesses y = x + h v + h^2 g (accepted from rest), the random one is rejected."""
+    rest, tets, mask, x0 = bar()
+    say("warm-started step vs oracle,", net, "network")
+    sc = SCALES.copy()
+    sc[6] = 2e-3
+    gp = P.default_params(precision=P.FP64, relgrad_tol=1e-4, max_iterations=5000)
+    s = P.Solver(rest, tets, mask, gp)
+    s.set_positions(rest if net == "zero" else x0)
+    s.learn_set_model(np.zeros(P.MLP_PARAMS, np.float32) if net == "zero" else model(), sc)
+    s.learn_enable(True)
+    o = O.Solver(rest, tets, mask, params=O.default_params(relgrad_tol=1e-4, iterations=5000), precision=64)
+    o.set_state(x=(rest if net == "zero" else x0).astype(np.float64))
+    h, mom = float(np.float32(0.03)), float(np.float32(0.2))
+    used = []
+    for k in range(4):
+        learned, plain = s.learn_guess()
+        xs, vs, vls = o.state()
+        assert np.abs(plain - (xs + h * vs + mom * (vs - vls) * h)).max() <= 1e-9
+        gs = s.step()
+        os_ = o.step(x0=learned)
+        say("step", k, "used gpu/oracle", gs.used_learned, os_.used_learned, "its", gs.iterations, os_.iterations,
+            "g_l/g0 %.3f" % (gs.g_learned / gs.g0_norm))
+        assert gs.used_learned == os_.used_learned
+        assert gs.g0_norm == pytest.approx(os_.g0_norm, rel=1e-6)
+        assert abs(gs.iterations - os_.iterations) <= 2
+        assert np.abs(s.positions(f64=True) - o.state()[0]).max() <= 1e-7 * 0.8
+        used.append(gs.used_learned)
+    assert (net == "zero") == any(used)
+    s.close()
+
+
+def test_collect_and_train_on_gpu():
+    rest, tets, mask, x0 = bar()
+    say("collect + train")
+    s = P.Solver(rest, tets, mask, P.default_params(precision=P.FP64, relgrad_tol=1e-4, max_iterations=5000))
+    s.set_positions(x0)
+    nfree = int((mask == 0).sum())
+    s.learn_collect(10 * nfree)
+    s.step()
+    v32, vl32 = s.velocities()
+    s.set_velocities(v32, vl32)  # the state's velocities are now exactly these FP32 values
+    s.learn_collect(10 * nfree)
+    x_before = s.positions(f64=True)
+    v_before = v32.astype(np.float64)
+    s.step()
+    x_after = s.positions(f64=True)
+    assert s.learn_samples() == nfree
+    f, t = s.learn_get_samples()
+    # target = converged position - (x + h v + h^2 g), free nodes in node order
+    h = float(np.float32(0.03))
+    y = x_before + h * v_before + h * h * np.float32([0, -9.8, 0]).astype(np.float64)
+    ref = (x_after - y)[mask == 0]
+    assert np.abs(t - ref).max() <= 1e-6 * np.abs(ref).max() + 1e-12
+    for _ in range(9):
+        s.step()
+    assert s.learn_samples() == 10 * nfree
+    s.step()  # full: not collected
+    assert s.learn_samples() == 10 * nfree
+    s.learn_fit_scales()
+    s.learn_init_model(5)
+    loss = s.learn_train(8, batch=512, lr=3e-3, seed=2)
+    say("loss per epoch", np.round(loss, 5))
+    assert loss[-1] < 0.5 * loss[0]
+    # the trained model's loss on the samples, through the C ABI, equals the numpy one
+    p, sc = s.learn_get_model()
+    lg, _ = s.learn_loss_grad(f, t)
+    lo, _ = MO.loss_grad(p, sc, f, t)
+    assert lg == pytest.approx(lo, rel=1e-4)
+    s.close()
+
+
+def test_trained_warm_start_gain():
+    """What the trained network gains: train on a seeded trajectory (twist, impulses), then run a
+    different seed with and without the warm start at rel-grad 1e-4 and report the iterations.
+    The gradient test guarantees that a worse guess is never used."""
+    rest, tets = M.bar(8, 8, 32, 0.05)
+    mask = M.clamp_x0(rest)
+    prm = P.default_params(precision=P.FP64, relgrad_tol=1e-4, max_iterations=20000, dt=1 / 30.0,
+                           grav=(0.0, 0.0, -9.8), profile=[(1, 0.96), (7, 0.991), (10, 0.9995)])
+
+    def run(seed, steps, mdl=None, collect=0):
+        s = P.Solver(rest, tets, mask, prm)
+        s.set_positions(M.twisted_state(rest, 8, 8, 32, 0.05, 60.0, 0.01, seed))
+        if collect:
+            s.learn_collect(collect)
+        if mdl is not None:
+            s.learn_set_model(*mdl)
+            s.learn_enable(True)
+            s.learn_set_options(100, (1.0, 0.5, 0.25, 0.1))
+        r = np.random.default_rng(seed)
+        st = []
+        for k in range(steps):
+            ext = None
+            if k % 10 == 5:
+                ext = np.zeros_like(rest)
+                ext[r.integers(0, len(rest), 10)] = r.normal(0, 20, (10, 3))
+            st.append(s.step(ext))
+        return s, st
+
+    s, _ = run(1, 60, collect=60 * len(rest))
+    s.learn_fit_scales()
+    s.learn_init_model(7)
+    loss = s.learn_train(20, batch=2048, lr=2e-3, seed=1)
+    mdl = s.learn_get_model()
+    s.close()
+    sp, stp = run(11, 30)
+    sl, stl = run(11, 30, mdl)
+    ip = np.mean([q.iterations for q in stp])
+    il = np.mean([q.iterations for q in stl])
+    used = np.mean([q.used_learned for q in stl])
+    say("trained: loss %.4f -> %.4f; iterations/step plain %.1f, warm-started %.1f (%.1f%%), guess used in %.0f%% of steps"
+        % (loss[0], loss[-1], ip, il, 100 * (ip - il) / ip, 100 * used))
+    assert loss[-1] < 0.2 * loss[0]
+    assert all(q.converged for q in stl)
+    # the warm start never costs more than a few percent (different trajectories round
+    # differently); what it gains is the reported number
+    assert il <= 1.03 * ip
+    sp.close()
+    sl.close()
