@@ -4,7 +4,12 @@ Default workload (one GPU): the 64x64x256 bar of 6,291,456 tets (Freudenthal spl
 neo-Hookean E=1e5 Pa, nu=0.45, rho=1000, gravity -z, clamp x=0), dt = 1/30, iterating each step
 until |g_k| <= 1e-4 |g_0| (FP64 state and per-tet arithmetic).  Prints one JSON line.
 
+In rel-grad mode the line also reports the learned warm start against the plain guess: the
+network is trained on the GPU on another seeded trajectory of the same bar (and of the
+12x12x50 bar, 43,200 tets, for the second comparison) and then steps the benchmark trajectory.
+
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2a|c2b|c1|c0] [--mode relgrad|fixed]
+                  [--no-learned] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -67,6 +72,83 @@ def cpu_baseline(rest, tets, mask, material, iters_per_step, grav, seconds=15.0)
             "iterations_per_s": its}
 
 
+WARM_OPTIONS = dict(sweeps=100, betas=(1.0, 0.5, 0.25, 0.1))
+
+
+def impulse(rest, rng, k):
+    """Seeded external impulses on 20 random nodes every 10th step of a training trajectory."""
+    if k % 10 != 5:
+        return None
+    ext = np.zeros_like(rest)
+    ext[rng.integers(0, len(rest), 20)] = rng.normal(0, 50.0, (20, 3))
+    return ext
+
+
+def train_warm_start(P, dims, params, steps, seed, epochs, capacity):
+    """The plain solver generates (state, converged step) pairs on a seeded trajectory (twist,
+    jitter, impulses) of the given bar; the network is trained on them on the GPU."""
+    nx, ny, nz = dims
+    rest, tets = P.mesh_bar(nx, ny, nz, 0.05)
+    mask = (rest[:, 0] == 0).astype(np.int32)
+    s = P.Solver(rest, tets, mask, params)
+    s.set_positions(P.mesh_twist(rest, nx, ny, nz, 0.05, 45.0, 0.01, seed))
+    s.learn_collect(capacity)
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        s.step(impulse(rest, rng, k))
+    t1 = time.perf_counter()
+    s.learn_fit_scales()
+    s.learn_init_model(seed)
+    loss = s.learn_train(epochs, batch=8192, lr=2e-3, seed=seed)
+    t2 = time.perf_counter()
+    model = s.learn_get_model()
+    info = {"trajectory": f"{nx}x{ny}x{nz} bar, 45 deg twist, 1% jitter, impulses, seed {seed}",
+            "steps": steps, "samples": s.learn_samples(), "epochs": epochs,
+            "loss_first_last": [float(loss[0]), float(loss[-1])], "collect_s": t1 - t0, "train_s": t2 - t1}
+    s.close()
+    return model, info
+
+
+def timed_steps(P, rest, tets, mask, x0, params, warmup, steps, model=None):
+    s = P.Solver(rest, tets, mask, params)
+    s.set_positions(x0)
+    if model is not None:
+        s.learn_set_model(*model)
+        s.learn_set_options(WARM_OPTIONS["sweeps"], WARM_OPTIONS["betas"])
+        s.learn_enable(True)
+    for _ in range(warmup):
+        s.step()
+    t0 = time.perf_counter()
+    st = [s.step() for _ in range(steps)]
+    dt = time.perf_counter() - t0
+    s.close()
+    return {"steps_per_s": steps / dt, "iterations_per_step": float(np.mean([q.iterations for q in st])),
+            "guess_used_fraction": float(np.mean([q.used_learned for q in st])),
+            "converged_steps": int(sum(q.converged for q in st))}
+
+
+def warm_start_report(P, args, params, rest, tets, mask, x0, dims, plain_sps, plain_its):
+    """Learned warm start vs plain: on the benchmark bar (trained on another trajectory of the same
+    bar) and on the 12x12x50 bar (43,200 tets)."""
+    out = {}
+    model, info = train_warm_start(P, dims, params, args.train_steps, 7, 8, 4_000_000)
+    lw = timed_steps(P, rest, tets, mask, x0, params, args.warmup, args.steps, model)
+    out["bench_bar"] = {"training": info, "plain_steps_per_s": plain_sps, "plain_iterations_per_step": plain_its,
+                        "learned": lw, "speedup": lw["steps_per_s"] / plain_sps}
+    c1 = (12, 12, 50)
+    r1, t1 = P.mesh_bar(*c1, 0.05)
+    m1 = (r1[:, 0] == 0).astype(np.int32)
+    x1 = P.mesh_twist(r1, *c1, 0.05, 90.0, 0.01, 11)
+    model1, info1 = train_warm_start(P, c1, params, 120, 1, 30, 2_000_000)
+    p1 = timed_steps(P, r1, t1, m1, x1, params, 1, 30)
+    l1 = timed_steps(P, r1, t1, m1, x1, params, 1, 30, model1)
+    out["bar_12x12x50"] = {"tets": len(t1), "training": info1, "plain": p1, "learned": l1,
+                           "speedup": l1["steps_per_s"] / p1["steps_per_s"]}
+    out["options"] = {"smoothing_sweeps": WARM_OPTIONS["sweeps"], "steps_tried": list(WARM_OPTIONS["betas"])}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +161,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--max-iterations", type=int, default=40000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
+    ap.add_argument("--train-steps", type=int, default=3, help="training-trajectory steps on the benchmark bar")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) for counter calibration")
     args = ap.parse_args()
@@ -163,9 +247,12 @@ def main():
                      "algorithmic_bytes_per_launch": alg[dom]},
         "learned_warm_start": None,
     }
+    s.close()
+    if relg and not args.no_learned:
+        out["learned_warm_start"] = warm_start_report(P, args, params, rest, tets, mask, x0, (nx, ny, nz),
+                                                      steps_per_s, iters / args.steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(rest, tets, mask, material, iters / args.steps, grav)
-    s.close()
     print(json.dumps(out))
 
 

@@ -69,7 +69,8 @@ typedef struct pbs_params {
   double relgrad_tol;   /* 0: fixed iterations; > 0: iterate until |g_k| <= tol |g_0| (free DOFs, FP64) */
   int max_iterations;   /* iteration cap in rel-grad mode */
   int block_size;       /* threads per workgroup of the per-tet / per-node utility kernels, 0 = 256 */
-  int tile_tets;        /* max tets per workgroup of the tile pass (16..4096), 0 = chosen from mesh size and LDS */
+  int tile_tets;        /* max tets per workgroup of the tile pass (16..4096); 0 = chosen from mesh size and
+                           LDS; -1 = autotuned at creation (64, 128, 256, 512 timed on 32 passes) */
   int graph_passes;     /* descent passes per captured HIP graph replay: 0 = 32, -1 = no graph
                            (kernels launched one by one) */
 } pbs_params;
@@ -103,6 +104,24 @@ int pbs_abi_version(void);
  * Rest tets must have det(Dm) > 0 (the AN convention of TetMesh.cpp:279-294). */
 int pbs_create(int num_nodes, const float *nodes, const float *rest, const int *constraint_mask, int num_tets,
                const int *tets, const pbs_params *p, pbs_solver **out);
+/* Batched solver (no reference counterpart): num_instances independent meshes stepped together
+ * (each tile and node pass covers all of them in one launch, with a loop control -- line search,
+ * Chebyshev, convergence -- per instance).  Instance i has inst_nodes[i] nodes and inst_tets[i]
+ * tets, concatenated in nodes / rest / constraint_mask / tets in instance order; tets hold
+ * indices into the concatenated nodes (inside the instance's range); youngs[i], nu[i] set the
+ * instance's material (p->youngs / p->nu are not used).  Results equal those of separate solvers
+ * with the same tile size.  The learned warm start needs a single-mesh solver. */
+int pbs_create_batch(int num_instances, const int *inst_nodes, const int *inst_tets, const float *nodes,
+                     const float *rest, const int *constraint_mask, const int *tets, const float *youngs,
+                     const float *nu, const pbs_params *p, pbs_solver **out);
+int pbs_num_instances(const pbs_solver *s);
+/* Loop statistics of one instance in the last pbs_step (gpu_ms and g_learned are 0).  pbs_step
+ * itself reports, for a batch, passes = the most any instance ran, iterations / rollbacks /
+ * converged summed, alpha the smallest, gradient norms combined as sqrt(sum of squares). */
+int pbs_instance_stats(const pbs_solver *s, int instance, pbs_step_stats *st);
+/* Tile sizes timed by the autotuner (pbs_params.tile_tets = -1) and their time per descent pass
+ * in microseconds; *count = 0 when the solver was not autotuned. */
+int pbs_autotune_report(const pbs_solver *s, int max, int *tile_tets, double *us_per_pass, int *count);
 /* ~CUDAInterface (CUDAInterface.cu:742-769). */
 void pbs_destroy(pbs_solver *s);
 /* Change solver parameters (material constants and gravity re-run the setup kernels). */

@@ -56,6 +56,10 @@ SIGNATURES = {
     "pbs_last_error": (C.c_char_p, []),
     "pbs_abi_version": (C.c_int, []),
     "pbs_create": (C.c_int, [C.c_int, _P, _P, _P, C.c_int, _P, _P, C.POINTER(_P)]),
+    "pbs_create_batch": (C.c_int, [C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(_P)]),
+    "pbs_num_instances": (C.c_int, [_P]),
+    "pbs_instance_stats": (C.c_int, [_P, C.c_int, _P]),
+    "pbs_autotune_report": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "pbs_destroy": (None, [_P]),
     "pbs_set_params": (C.c_int, [_P, _P]),
     "pbs_get_params": (C.c_int, [_P, _P]),
@@ -222,12 +226,64 @@ class Solver:
                                 C.byref(self.params), C.byref(h)))
         self.h = h
 
+    @classmethod
+    def batch(cls, meshes, youngs, nu, params=None, nodes=None):
+        """Batched solver over independent meshes: meshes = [(rest, tets, mask), ...] with tets
+        indexing each mesh's own nodes; youngs / nu per mesh.  Arrays of the solver (positions,
+        velocities, ...) are the meshes' arrays concatenated in order."""
+        self = cls.__new__(cls)
+        rests, tl, masks, nns, nts = [], [], [], [], []
+        off = 0
+        for rest, tets, mask in meshes:
+            r = _f32(rest, (-1, 3))
+            t = np.ascontiguousarray(tets, np.int32).reshape(-1, 4) + off
+            m = np.zeros(len(r), np.int32) if mask is None else np.ascontiguousarray(mask, np.int32)
+            rests.append(r)
+            tl.append(t)
+            masks.append(m)
+            nns.append(len(r))
+            nts.append(len(t))
+            off += len(r)
+        self.rest = np.concatenate(rests)
+        self.tets = np.concatenate(tl)
+        self.mask = np.concatenate(masks)
+        self.nn, self.nt = len(self.rest), len(self.tets)
+        self.inst_nodes = np.array(nns, np.int32)
+        self.inst_tets = np.array(nts, np.int32)
+        self.params = params if params is not None else default_params()
+        E = np.ascontiguousarray(youngs, np.float32)
+        v = np.ascontiguousarray(nu, np.float32)
+        x0 = None if nodes is None else _f32(nodes, (-1, 3))
+        h = C.c_void_p()
+        _check(lib().pbs_create_batch(len(meshes), _p(self.inst_nodes), _p(self.inst_tets), _p(x0), _p(self.rest),
+                                      _p(self.mask), _p(self.tets), _p(E), _p(v), C.byref(self.params), C.byref(h)))
+        self.h = h
+        return self
+
+    def instance_stats(self, i):
+        st = StepStats()
+        _check(lib().pbs_instance_stats(self.h, i, C.byref(st)))
+        return st
+
+    def autotune_report(self):
+        n = 16
+        tiles = np.zeros(n, np.int32)
+        us = np.zeros(n)
+        c = C.c_int()
+        _check(lib().pbs_autotune_report(self.h, n, _p(tiles), _p(us), C.byref(c)))
+        return {int(t): float(u) for t, u in zip(tiles[:c.value], us[:c.value])}
+
     def close(self):
         if getattr(self, "h", None):
             lib().pbs_destroy(self.h)
             self.h = None
 
     __del__ = close
+
+    def get_params(self):
+        p = Params()
+        _check(lib().pbs_get_params(self.h, C.byref(p)))
+        return p
 
     def set_params(self, params):
         _check(lib().pbs_set_params(self.h, C.byref(params)))

@@ -42,8 +42,8 @@ static int validate_params(const pbs_params *p) {
   if (p->relgrad_tol < 0 || (p->relgrad_tol > 0 && p->max_iterations <= 0)) return set_error(PBS_ERR_ARG, "bad rel-grad settings");
   if (p->block_size != 0 && p->block_size != 64 && p->block_size != 128 && p->block_size != 256)
     return set_error(PBS_ERR_ARG, "block_size must be 0, 64, 128 or 256");
-  if (p->tile_tets != 0 && (p->tile_tets < 16 || p->tile_tets > 4096))
-    return set_error(PBS_ERR_ARG, "tile_tets must be 0 or in [16, 4096]");
+  if (p->tile_tets != 0 && p->tile_tets != -1 && (p->tile_tets < 16 || p->tile_tets > 4096))
+    return set_error(PBS_ERR_ARG, "tile_tets must be 0, -1 or in [16, 4096]");
   if (p->graph_passes < -1 || p->graph_passes > 1024) return set_error(PBS_ERR_ARG, "graph_passes must be in [-1, 1024]");
   return PBS_OK;
 }
@@ -151,6 +151,20 @@ template <typename R> static int reset_impl(pbs_solver *s) {
   return PBS_OK;
 }
 
+// Per-instance Lame constants, with the reference's formulas (CUDAInterface.cu:704-705) in the
+// same arithmetic as the single-mesh constants of make_devparams.
+static int upload_lame(pbs_solver *s) {
+  std::vector<float> f(2 * (size_t)s->ninst);
+  for (int i = 0; i < s->ninst; ++i) {
+    const float E = s->inst_youngs[i], nu = s->inst_nu[i];
+    const float mu = (float)(0.5 * (double)E / (double)(1 + nu));
+    f[i] = mu;
+    f[s->ninst + i] = mu * 2 / (1 - 2 * nu);
+  }
+  HIPCHK(hipMemcpyAsync((void *)s->td.inst_mu, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice, s->st));
+  return PBS_OK;
+}
+
 template <typename R> static int alloc_impl(pbs_solver *s) {
   DevMesh<R> &m = mesh<R>(s);
   const size_t N = s->nn, T = std::max(s->nt, 1);
@@ -192,7 +206,7 @@ template <typename R> static int alloc_impl(pbs_solver *s) {
   rc |= dalloc(s, &bnd_off, N + 1);
   rc |= dalloc(s, &hslot, H);
   rc |= dalloc(s, &s->td.tpart, (size_t)P.ntiles);
-  rc |= dalloc(s, &s->td.npart, 2 * (N / 64 + 2));
+  rc |= dalloc(s, &s->td.npart, 2 * (size_t)std::max(s->td.nblk, 1) + 2);
   rc |= dalloc(s, &m.fown, 3 * N);
   rc |= dalloc(s, &m.hpart, 3 * H);
   rc |= dalloc(s, &m.fownh, 3 * N);
@@ -203,7 +217,22 @@ template <typename R> static int alloc_impl(pbs_solver *s) {
   s->td.tile_off = toff;
   s->td.bnd_off = bnd_off;
   s->td.halo_slot = hslot;
-  rc |= dalloc(s, &m.ctrl, 1);
+  rc |= dalloc(s, &m.ctrl, (size_t)s->ninst);
+  {
+    int *a;
+    float *f;
+    const size_t I = (size_t)s->ninst;
+    rc |= dalloc(s, &a, 4 * (I + 1) + 2 * (size_t)std::max(s->td.nblk, 1));
+    s->td.inst_node_off = a;
+    s->td.inst_tile_off = a + (I + 1);
+    s->td.inst_tet_off = a + 2 * (I + 1);
+    s->td.inst_blk_off = a + 3 * (I + 1);
+    s->td.blk_inst = a + 4 * (I + 1);
+    s->td.blk_start = a + 4 * (I + 1) + std::max(s->td.nblk, 1);
+    rc |= dalloc(s, &f, 2 * I);
+    s->td.inst_mu = f;
+    s->td.inst_lam = f + I;
+  }
   rc |= dalloc(s, &m.flag, 1);
   rc |= dalloc(s, &m.tete, T);
   rc |= dalloc(s, &s->stage, 3 * N);
@@ -228,6 +257,27 @@ template <typename R> static int setup_impl(pbs_solver *s, const int *mask) {
   HIPCHK(hipMemcpyAsync(m.inc_off, P.inc_off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, s->st));
   if (s->nt > 0) HIPCHK(hipMemcpyAsync(m.inc, P.inc.data(), sizeof(int) * 4 * (size_t)s->nt, hipMemcpyHostToDevice, s->st));
   HIPCHK(hipMemcpyAsync((void *)s->td.tiles, P.tiles.data(), sizeof(TileInfo) * P.tiles.size(), hipMemcpyHostToDevice, s->st));
+  {
+    const size_t I = (size_t)s->ninst;
+    std::vector<int> a(4 * (I + 1) + 2 * (size_t)std::max(s->td.nblk, 1), 0);
+    for (size_t i = 0; i <= I; ++i) {
+      a[i] = P.inst_node_off[i];
+      a[(I + 1) + i] = P.inst_tile_off[i];
+      a[2 * (I + 1) + i] = P.inst_tet_off[i];
+    }
+    int b = 0;
+    for (size_t i = 0; i < I; ++i) {
+      a[3 * (I + 1) + i] = b;
+      for (int n = P.inst_node_off[i]; n < P.inst_node_off[i + 1]; n += 256, ++b) {
+        a[4 * (I + 1) + b] = (int)i;
+        a[4 * (I + 1) + std::max(s->td.nblk, 1) + b] = n;
+      }
+    }
+    a[3 * (I + 1) + I] = b;
+    HIPCHK(hipMemcpyAsync((void *)s->td.inst_node_off, a.data(), sizeof(int) * a.size(), hipMemcpyHostToDevice, s->st));
+    int rc = upload_lame(s);
+    if (rc) return rc;
+  }
   if (!P.halo.empty())
     HIPCHK(hipMemcpyAsync((void *)s->td.halo, P.halo.data(), sizeof(int) * P.halo.size(), hipMemcpyHostToDevice, s->st));
   if (s->nt > 0)
@@ -314,18 +364,18 @@ template <typename R> static int set_positions_impl(pbs_solver *s, const float *
   return PBS_OK;
 }
 
-extern "C" int pbs_create(int nn, const float *nodes, const float *rest, const int *mask, int nt, const int *tets,
-                          const pbs_params *p, pbs_solver **out) {
-  if (!out) return set_error(PBS_ERR_ARG, "null out");
-  *out = nullptr;
-  if (nn <= 0 || nt < 0 || !rest || (nt > 0 && !tets)) return set_error(PBS_ERR_ARG, "pbs_create: bad mesh arguments");
-  int rc = validate_params(p);
-  if (rc) return rc;
-  for (long i = 0; i < 4L * nt; ++i)
-    if (tets[i] < 0 || tets[i] >= nn) return set_error(PBS_ERR_MESH, "tet index out of range");
+// Creates a solver for ninst independent meshes (1 for pbs_create): instance i owns nodes
+// [node_off[i], node_off[i+1]) and tets [tet_off[i], tet_off[i+1]), tets hold global node ids.
+static int create_impl(int ninst, const int *node_off, const int *tet_off, const float *nodes, const float *rest,
+                       const int *mask, const int *tets, const float *youngs, const float *nu, const pbs_params *p,
+                       int tile_tets, pbs_solver **out) {
+  const int nn = node_off[ninst], nt = tet_off[ninst];
   pbs_solver *s = new pbs_solver();
   s->nn = nn;
   s->nt = nt;
+  s->ninst = ninst;
+  s->inst_youngs.assign(youngs, youngs + ninst);
+  s->inst_nu.assign(nu, nu + ninst);
   s->p = *p;
   make_devparams(s);
   s->rest_h.assign(rest, rest + 3 * (size_t)nn);
@@ -350,23 +400,28 @@ extern "C" int pbs_create(int nn, const float *nodes, const float *rest, const i
     t.tcap = std::max(tcap, 1);
     return f64 ? Launch<double>::tile_lds(t) : Launch<float>::tile_lds(t);
   };
-  int tt = p->tile_tets;
+  int tt = tile_tets;
   if (tt == 0) {
     tt = 256;
     while (tt > 64 && (nt / tt < 1024 || lds_need(std::max(64, tt / 2), tt) > std::min<size_t>(s->lds_budget, 40960))) tt /= 2;
   }
   std::string msg;
-  if (!build_tile_plan(nn, rest, nt, tets, tt, s->plan, msg)) return fail(set_error(PBS_ERR_MESH, msg));
+  if (!build_batch_plan(ninst, node_off, tet_off, rest, tets, tt, s->plan, msg)) return fail(set_error(PBS_ERR_MESH, msg));
   if (lds_need(s->plan.lcap, s->plan.tcap) > s->lds_budget)
     return fail(set_error(PBS_ERR_ARG, "tile too large for the LDS of one workgroup: use a smaller tile_tets"));
+  s->p.tile_tets = tt;
   s->td.ntiles = s->plan.ntiles;
   s->td.lcap = std::max(s->plan.lcap, 1);
   s->td.tcap = std::max(s->plan.tcap, 1);
   s->td.halo_total = s->plan.halo_total;
+  s->td.ninst = ninst;
+  s->td.nblk = 0;
+  for (int i = 0; i < ninst; ++i) s->td.nblk += (node_off[i + 1] - node_off[i] + 255) / 256;
+  s->ctrl_h.assign(ninst, Ctrl{});
   if (hipStreamCreate(&s->st) != hipSuccess) return fail(set_error(PBS_ERR_HIP, "hipStreamCreate failed"));
   if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
     return fail(set_error(PBS_ERR_HIP, "hipEventCreate failed"));
-  rc = p->precision == PBS_FP64 ? alloc_impl<double>(s) : alloc_impl<float>(s);
+  int rc = p->precision == PBS_FP64 ? alloc_impl<double>(s) : alloc_impl<float>(s);
   if (rc) return fail(rc);
   rc = p->precision == PBS_FP64 ? setup_impl<double>(s, mask) : setup_impl<float>(s, mask);
   if (rc) return fail(rc);
@@ -375,6 +430,126 @@ extern "C" int pbs_create(int nn, const float *nodes, const float *rest, const i
     if (rc) return fail(rc);
   }
   *out = s;
+  return PBS_OK;
+}
+
+// Autotuner: builds the solver with each candidate tile size, times 32 fixed descent passes from
+// the initial state (after 8 untimed ones) and keeps the fastest.
+static int create_tuned(int ninst, const int *node_off, const int *tet_off, const float *nodes, const float *rest,
+                        const int *mask, const int *tets, const float *youngs, const float *nu, const pbs_params *p,
+                        pbs_solver **out) {
+  static const int cand[] = {64, 128, 256, 512};
+  std::vector<int> tiles;
+  std::vector<double> us;
+  int best = -1;
+  double best_us = 0.0;
+  for (int tt : cand) {
+    pbs_solver *s = nullptr;
+    pbs_params q = *p;
+    q.iterations = 8;
+    q.relgrad_tol = 0.0;
+    if (create_impl(ninst, node_off, tet_off, nodes, rest, mask, tets, youngs, nu, &q, tt, &s)) continue;
+    pbs_step_stats st{};
+    int rc = pbs_step(s, nullptr, nullptr, &st);
+    q.iterations = 32;
+    if (!rc) rc = pbs_set_params(s, &q);
+    if (!rc) rc = pbs_reset(s);
+    if (!rc && nodes) rc = pbs_set_positions(s, nodes);
+    if (!rc) rc = pbs_step(s, nullptr, nullptr, &st);
+    pbs_destroy(s);
+    if (rc || st.passes <= 0) continue;
+    const double t = 1e3 * st.gpu_ms / st.passes;
+    tiles.push_back(tt);
+    us.push_back(t);
+    if (best < 0 || t < best_us) {
+      best = tt;
+      best_us = t;
+    }
+  }
+  if (best < 0) return set_error(PBS_ERR_ARG, "autotuner: no tile size fits this mesh");
+  int rc = create_impl(ninst, node_off, tet_off, nodes, rest, mask, tets, youngs, nu, p, best, out);
+  if (rc) return rc;
+  (*out)->tune_tiles = tiles;
+  (*out)->tune_us = us;
+  return PBS_OK;
+}
+
+static int create_any(int ninst, const int *node_off, const int *tet_off, const float *nodes, const float *rest,
+                      const int *mask, const int *tets, const float *youngs, const float *nu, const pbs_params *p,
+                      pbs_solver **out) {
+  if (p->tile_tets < 0) return create_tuned(ninst, node_off, tet_off, nodes, rest, mask, tets, youngs, nu, p, out);
+  return create_impl(ninst, node_off, tet_off, nodes, rest, mask, tets, youngs, nu, p, p->tile_tets, out);
+}
+
+extern "C" int pbs_create(int nn, const float *nodes, const float *rest, const int *mask, int nt, const int *tets,
+                          const pbs_params *p, pbs_solver **out) {
+  if (!out) return set_error(PBS_ERR_ARG, "null out");
+  *out = nullptr;
+  if (nn <= 0 || nt < 0 || !rest || (nt > 0 && !tets)) return set_error(PBS_ERR_ARG, "pbs_create: bad mesh arguments");
+  int rc = validate_params(p);
+  if (rc) return rc;
+  for (long i = 0; i < 4L * nt; ++i)
+    if (tets[i] < 0 || tets[i] >= nn) return set_error(PBS_ERR_MESH, "tet index out of range");
+  const int node_off[2] = {0, nn}, tet_off[2] = {0, nt};
+  return create_any(1, node_off, tet_off, nodes, rest, mask, tets, &p->youngs, &p->nu, p, out);
+}
+
+extern "C" int pbs_create_batch(int ninst, const int *inst_nodes, const int *inst_tets, const float *nodes,
+                                const float *rest, const int *mask, const int *tets, const float *youngs,
+                                const float *nu, const pbs_params *p, pbs_solver **out) {
+  if (!out) return set_error(PBS_ERR_ARG, "null out");
+  *out = nullptr;
+  if (ninst <= 0 || !inst_nodes || !inst_tets || !rest || !youngs || !nu)
+    return set_error(PBS_ERR_ARG, "pbs_create_batch: bad arguments");
+  int rc = validate_params(p);
+  if (rc) return rc;
+  std::vector<int> node_off(ninst + 1, 0), tet_off(ninst + 1, 0);
+  for (int i = 0; i < ninst; ++i) {
+    if (inst_nodes[i] <= 0 || inst_tets[i] < 0) return set_error(PBS_ERR_ARG, "instance sizes must be positive");
+    if (!(youngs[i] > 0) || !(nu[i] > -1.0f && nu[i] < 0.5f)) return set_error(PBS_ERR_ARG, "bad instance material");
+    if ((long)node_off[i] + inst_nodes[i] > 2147483647L || (long)tet_off[i] + inst_tets[i] > 2147483647L)
+      return set_error(PBS_ERR_ARG, "batch too large");
+    node_off[i + 1] = node_off[i] + inst_nodes[i];
+    tet_off[i + 1] = tet_off[i] + inst_tets[i];
+  }
+  if (tet_off[ninst] > 0 && !tets) return set_error(PBS_ERR_ARG, "null tets");
+  for (int i = 0; i < ninst; ++i)
+    for (long k = 4L * tet_off[i]; k < 4L * tet_off[i + 1]; ++k)
+      if (tets[k] < node_off[i] || tets[k] >= node_off[i + 1])
+        return set_error(PBS_ERR_MESH, "tet index outside its instance's node range");
+  return create_any(ninst, node_off.data(), tet_off.data(), nodes, rest, mask, tets, youngs, nu, p, out);
+}
+
+extern "C" int pbs_num_instances(const pbs_solver *s) {
+  if (!s) return set_error(PBS_ERR_ARG, "null handle");
+  return s->ninst;
+}
+
+extern "C" int pbs_instance_stats(const pbs_solver *s, int inst, pbs_step_stats *st) {
+  if (!s || !st) return set_error(PBS_ERR_ARG, "null argument");
+  if (inst < 0 || inst >= s->ninst) return set_error(PBS_ERR_ARG, "instance out of range");
+  const Ctrl &c = s->ctrl_h[inst];
+  *st = pbs_step_stats{};
+  st->passes = c.passes;
+  st->iterations = c.k;
+  st->rollbacks = c.rollbacks;
+  st->converged = c.converged;
+  st->used_learned = c.used_learned;
+  st->alpha = c.alpha;
+  st->energy = c.E_old;
+  st->g0_norm = c.g0;
+  st->g_norm = c.g;
+  return PBS_OK;
+}
+
+extern "C" int pbs_autotune_report(const pbs_solver *s, int max, int *tile_tets, double *us_per_pass, int *count) {
+  if (!s || !count) return set_error(PBS_ERR_ARG, "null argument");
+  const int n = (int)s->tune_tiles.size();
+  for (int i = 0; i < std::min(n, max); ++i) {
+    if (tile_tets) tile_tets[i] = s->tune_tiles[i];
+    if (us_per_pass) us_per_pass[i] = s->tune_us[i];
+  }
+  *count = n;
   return PBS_OK;
 }
 
@@ -402,8 +577,16 @@ extern "C" int pbs_set_params(pbs_solver *s, const pbs_params *p) {
   int rc = validate_params(p);
   if (rc) return rc;
   if (p->precision != s->p.precision) return set_error(PBS_ERR_STATE, "precision is fixed at creation");
+  const int tile = s->p.tile_tets;  // the tile plan is fixed at creation
   s->p = *p;
+  s->p.tile_tets = tile;
   make_devparams(s);
+  if (s->ninst == 1) {  // a batched solver keeps its per-instance materials
+    s->inst_youngs[0] = p->youngs;
+    s->inst_nu[0] = p->nu;
+    rc = upload_lame(s);
+    if (rc) return rc;
+  }
   if (s->p.precision == PBS_FP64) {
     Launch<double>::setup_nodes(s->m64, s->dp, 256, s->st);
   } else {
@@ -558,7 +741,7 @@ template <typename R> static int step_impl(pbs_solver *s, const float *ext, floa
   HIPCHK(hipEventRecord(s->ev0, s->st));
   s->pass_ordinal = -1;
   int i = timed_begin(s, K_INIT);
-  Launch<R>::step_init(m, s->dp, s->bs, s->st);
+  Launch<R>::step_init(m, s->td, s->dp, s->bs, s->st);
   timed_end(s, i);
   HIPCHK(hipGetLastError());
   if (s->learn) {  // features / learned guess / selection by gradient norm
@@ -571,20 +754,37 @@ template <typename R> static int step_impl(pbs_solver *s, const float *ext, floa
   const long cap = relg ? (long)s->p.max_iterations + 256 : (long)s->p.iterations + 256;
   long launched = 0;
   long n = relg ? 32 : std::max(s->p.iterations, 1);
-  Ctrl c{};
+  // control blocks of all instances: the loop runs while any instance is active; a pass counts
+  // as executed when some instance ran it
+  std::vector<Ctrl> &ch = s->ctrl_h;
+  auto read_ctrl = [&]() -> int {
+    HIPCHK(hipMemcpyAsync(ch.data(), m.ctrl, sizeof(Ctrl) * ch.size(), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return PBS_OK;
+  };
+  auto max_passes = [&]() {
+    int mp = 0;
+    for (const Ctrl &q : ch) mp = std::max(mp, q.passes);
+    return mp;
+  };
+  auto all_done = [&]() {
+    for (const Ctrl &q : ch)
+      if (!q.done) return false;
+    return true;
+  };
   while (true) {
     if (use_graph) {
       const long reps = (n + G - 1) / G;
       for (long r = 0; r < reps; ++r) {
         const bool timed = period && (s->prof_tick++ % period) == 0;
         if (timed) {  // the events are reused by the next timed replay: collect after this one
-          HIPCHK(hipMemcpyAsync(&c, m.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->st));
-          HIPCHK(hipStreamSynchronize(s->st));
-          const int before = c.passes;
+          int rc = read_ctrl();
+          if (rc) return rc;
+          const int before = max_passes();
           HIPCHK(hipGraphLaunch(s->graph[1].exec, s->st));
-          HIPCHK(hipMemcpyAsync(&c, m.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->st));
-          HIPCHK(hipStreamSynchronize(s->st));
-          graph_collect(s, c.passes - before);
+          rc = read_ctrl();
+          if (rc) return rc;
+          graph_collect(s, max_passes() - before);
         } else {
           HIPCHK(hipGraphLaunch(s->graph[0].exec, s->st));
         }
@@ -598,15 +798,17 @@ template <typename R> static int step_impl(pbs_solver *s, const float *ext, floa
       HIPCHK(hipGetLastError());
       launched += n;
     }
-    HIPCHK(hipMemcpyAsync(&c, m.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->st));
-    HIPCHK(hipStreamSynchronize(s->st));
-    if (c.done) break;
+    int rc = read_ctrl();
+    if (rc) return rc;
+    if (all_done()) break;
     if (launched >= cap) return set_error(PBS_ERR_STATE, "descent loop did not terminate within the pass cap");
-    n = relg ? 32 : std::max(1, c.budget - c.k) + 1;
+    n = 1;
+    for (const Ctrl &q : ch)
+      if (!q.done) n = std::max<long>(n, relg ? 32 : std::max(1, q.budget - q.k) + 1);
   }
   s->pass_ordinal = -1;
   i = timed_begin(s, K_FINAL);
-  Launch<R>::step_final(m, s->dp, s->bs, s->st);
+  Launch<R>::step_final(m, s->td, s->dp, s->st);
   timed_end(s, i);
   HIPCHK(hipGetLastError());
   if (s->learn) {
@@ -619,19 +821,29 @@ template <typename R> static int step_impl(pbs_solver *s, const float *ext, floa
     HIPCHK(hipMemcpyAsync(out, s->stage, sizeof(float) * 3 * (size_t)s->nn, hipMemcpyDeviceToHost, s->st));
   }
   HIPCHK(hipStreamSynchronize(s->st));
-  if (period) timed_collect(s, c.passes);
+  if (period) timed_collect(s, max_passes());
   if (stats) {
+    // one instance: its loop statistics; several: passes = the most any instance ran, the counts
+    // summed, alpha the smallest, energy summed, gradient norms combined as sqrt(sum of squares)
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
-    stats->passes = c.passes;
-    stats->iterations = c.k;
-    stats->rollbacks = c.rollbacks;
-    stats->converged = c.converged;
-    stats->used_learned = c.used_learned;
+    const Ctrl &c = ch[0];
+    *stats = pbs_step_stats{};
+    stats->passes = max_passes();
     stats->alpha = c.alpha;
-    stats->energy = c.E_old;
-    stats->g0_norm = c.g0;
-    stats->g_norm = c.g;
+    double g0 = 0.0, g = 0.0;
+    for (const Ctrl &q : ch) {
+      stats->iterations += q.k;
+      stats->rollbacks += q.rollbacks;
+      stats->converged += q.converged;
+      stats->used_learned += q.used_learned;
+      stats->alpha = std::min(stats->alpha, q.alpha);
+      stats->energy += q.E_old;
+      g0 += q.g0 * q.g0;
+      g += q.g * q.g;
+    }
+    stats->g0_norm = s->ninst == 1 ? c.g0 : std::sqrt(g0);
+    stats->g_norm = s->ninst == 1 ? c.g : std::sqrt(g);
     stats->gpu_ms = ms;
     stats->g_learned = c.used_learned || (s->learn && s->learn->enabled) ? std::sqrt(c.g_learned) : 0.0;
   }
@@ -677,7 +889,7 @@ template <typename R> static int forces_impl(pbs_solver *s, const float *pos, fl
   int rc = upload_planes<R>(s, pos, m.ring.b[0]);
   if (rc) return rc;
   Launch<R>::standalone_ctrl(m, 0, s->st);
-  Launch<R>::tet(m, s->dp, s->bs, 0, 1, s->st);
+  Launch<R>::tet(m, s->td, s->dp, s->bs, 0, 1, s->st);
   HIPCHK(hipGetLastError());
   if (nf) {
     Launch<R>::gather(m, s->dp, s->bs, 0, m.ring.b[1], s->st);
@@ -706,7 +918,7 @@ template <typename R> static int hd_impl(pbs_solver *s, const float *pos, float 
   int rc = upload_planes<R>(s, pos, m.ring.b[0]);
   if (rc) return rc;
   Launch<R>::standalone_ctrl(m, 0, s->st);
-  Launch<R>::tet(m, s->dp, s->bs, 1, 0, s->st);
+  Launch<R>::tet(m, s->td, s->dp, s->bs, 1, 0, s->st);
   HIPCHK(hipGetLastError());
   if (nh) {
     Launch<R>::gather(m, s->dp, s->bs, 1, m.hd, s->st);

@@ -44,6 +44,12 @@ struct Ctrl {
 // Tile partition on the device (see pbs_tiles.h for the host-side construction).
 struct TileDev {
   int ntiles, lcap, tcap, pad;
+  // instances (independent meshes; 1 unless batched): node / tile / tet ranges, node blocks of
+  // 256 nodes per instance (block b: instance blk_inst[b], first node blk_start[b]), Lame constants
+  int ninst, nblk;
+  const int *inst_node_off, *inst_tile_off, *inst_tet_off, *inst_blk_off;
+  const int *blk_inst, *blk_start;
+  const float *inst_mu, *inst_lam;
   long halo_total;
   const TileInfo *tiles;
   const int *halo;
@@ -97,14 +103,15 @@ template <typename R> struct DevMesh {
 template <typename R> struct Launch {
   static void setup_tets(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st);
   static void setup_nodes(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st);
-  static void tet(const DevMesh<R> &m, const DevParams &p, int bs, int force_hd, int write_e, hipStream_t st);
+  static void tet(const DevMesh<R> &m, const TileDev &td, const DevParams &p, int bs, int force_hd, int write_e,
+                  hipStream_t st);
   static size_t tile_lds(const TileDev &td);
   static void tile_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
   static void node_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
   static void grad_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
   static void control(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
-  static void step_init(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st);
-  static void step_final(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st);
+  static void step_init(const DevMesh<R> &m, const TileDev &td, const DevParams &p, int bs, hipStream_t st);
+  static void step_final(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
   static void standalone_ctrl(const DevMesh<R> &m, int cur, hipStream_t st);
   static void gather(const DevMesh<R> &m, const DevParams &p, int bs, int which, void *out_planes, hipStream_t st);
   static void aos_to_planes(const float *aos, R *planes, const int *perm, int n, hipStream_t st);

@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(256) k_tet(DevParams p, const Ctrl *__restrict
                                              const int4 *__restrict__ tets, const float *__restrict__ dmi,
                                              const float *__restrict__ vol, R *__restrict__ frec,
                                              float *__restrict__ hrec, double *__restrict__ etet,
-                                             R *__restrict__ tete, int force_hd) {
+                                             R *__restrict__ tete, int force_hd, TileDev td) {
   __shared__ double lds[16];
   if (ctrl->done) return;
   const int hd = force_hd >= 0 ? force_hd : ctrl->hd_now;
@@ -152,7 +152,13 @@ __global__ void __launch_bounds__(256) k_tet(DevParams p, const Ctrl *__restrict
     R F[9];
     mul3(Ds, Dmi, F);
     R f12[12], h12[12], en;
-    const R mu = (R)p.mu, lam = (R)p.lam;
+    int lo = 0, hi = td.ninst;  // instance of tet t
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (td.inst_tet_off[mid] <= t) lo = mid;
+      else hi = mid;
+    }
+    const R mu = (R)td.inst_mu[lo], lam = (R)td.inst_lam[lo];
     if (hd) {
       if (MAT == 0) neo_tet<R, true>(F, Dmi, V, mu, lam, f12, en, h12);
       else stretch_tet<R, true>(F, Dmi, V, mu, lam, f12, en, h12);
@@ -263,11 +269,12 @@ k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
   constexpr int W = ContribWidth<R>::value;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double lds_red[16];
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const TileInfo ti = td.tiles[b];
+  ctrl += ti.inst;
   if (ctrl->done) return;
   const int hdnow = ctrl->hd_now, chk = ctrl->check_now;
   const R *__restrict__ x = ring.b[ctrl->cur];
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const TileInfo ti = td.tiles[b];
   const int L = ti.own_cnt + ti.halo_cnt, N = p.nn, T = p.nt, tid = threadIdx.x;
   R *sx = (R *)smem;
   R *rec = sx + 4 * td.lcap;
@@ -278,7 +285,7 @@ k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
     sx[4 * i + 2] = x[2 * (size_t)N + g];
   }
   __syncthreads();
-  const R mu = (R)p.mu, lam = (R)p.lam;
+  const R mu = (R)td.inst_mu[ti.inst], lam = (R)td.inst_lam[ti.inst];
   double e_el = 0.0;
   for (int phase = 0; phase <= hdnow; ++phase) {
     for (int s = tid; s < ti.tet_cnt; s += 256) {
@@ -389,21 +396,18 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
        float *__restrict__ hd, const float *__restrict__ mass, const int *__restrict__ mask,
        const float *__restrict__ ext, const R *__restrict__ xt) {
   __shared__ double lds_red[16];
+  const int inst = td.blk_inst[blockIdx.x];
+  ctrl += inst;
   if (ctrl->done) return;
   const int hdnow = ctrl->hd_now, cheb = ctrl->cheb_now;
   const R alpha = (R)ctrl->alpha, om = (R)ctrl->omega_next;
   const R *__restrict__ x = ring.b[ctrl->cur];
   const R *__restrict__ xl = ring.b[ctrl->last];
   R *__restrict__ xn = ring.b[ctrl->nxt];
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = td.blk_start[blockIdx.x] + (int)threadIdx.x;
   const int N = p.nn;
   double en = 0.0, g2 = 0.0;
-  if (!GRAD_ONLY && ctrl->check_now) {  // this block's share of the tiles' elastic energies, in tile order
-    const int per = (td.ntiles + gridDim.x - 1) / gridDim.x, t0 = blockIdx.x * per;
-    const int t1 = min(td.ntiles, t0 + per);
-    for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x) en += td.tpart[t];
-  }
-  if (n < N) {
+  if (n < td.inst_node_off[inst + 1]) {
     R f[3];
     float hv[3];
 #pragma unroll
@@ -470,14 +474,19 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
   }
 }
 
-// Control step: FP64 sums of the node blocks' partials (energy, |g|^2) in a fixed order, then the
-// loop control of the reference.
-__global__ void __launch_bounds__(1024) k_control(DevParams p, Ctrl *__restrict__ ctrl, TileDev td, int nnb) {
+// Control step, one workgroup per instance: FP64 sums in a fixed order of the instance's tile
+// energies (check passes) and node-block partials (energy, |g|^2), then the loop control of the
+// reference.
+__global__ void __launch_bounds__(1024) k_control(DevParams p, Ctrl *__restrict__ ctrl, TileDev td) {
   __shared__ double la[1024], lc[1024];
+  const int inst = blockIdx.x;
+  ctrl += inst;
   if (ctrl->done) return;
   const int tid = threadIdx.x;
   double a = 0.0, c = 0.0;
-  for (int i = tid; i < nnb; i += 1024) {
+  if (ctrl->check_now)
+    for (int i = td.inst_tile_off[inst] + tid; i < td.inst_tile_off[inst + 1]; i += 1024) a += td.tpart[i];
+  for (int i = td.inst_blk_off[inst] + tid; i < td.inst_blk_off[inst + 1]; i += 1024) {
     a += td.npart[2 * i];
     c += td.npart[2 * i + 1];
   }
@@ -503,7 +512,7 @@ __global__ void __launch_bounds__(1024) k_control(DevParams p, Ctrl *__restrict_
 // device loop state reset.
 // ---------------------------------------------------------------------------------------------
 template <typename R>
-__global__ void k_step_init(DevParams p, Ctrl *__restrict__ ctrl, Ring<R> ring, const R *__restrict__ xpos,
+__global__ void k_step_init(DevParams p, Ctrl *__restrict__ ctrl, int ninst, Ring<R> ring, const R *__restrict__ xpos,
                             const R *__restrict__ v, const R *__restrict__ vl, R *__restrict__ xt) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n < p.nn) {
@@ -520,7 +529,7 @@ __global__ void k_step_init(DevParams p, Ctrl *__restrict__ ctrl, Ring<R> ring, 
       ring.b[0][i] = x0;
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (n < ninst) {  // one control block per instance
     Ctrl C;
     C.k = 0;
     C.budget = p.iterations;
@@ -552,19 +561,20 @@ __global__ void k_step_init(DevParams p, Ctrl *__restrict__ ctrl, Ring<R> ring, 
     C.g_plain = 0.0;
     C.g_learned = 0.0;
     C.done = (p.iterations <= 0 || p.alpha0 < p.alpha_min) ? 1 : 0;
-    *ctrl = C;
+    ctrl[n] = C;
   }
 }
 
 // Step end: velocities (updateVelocity, CUDAInterface.cu:625-639, or DescentOptimize.cpp:183-185)
 // and the new positions.
 template <typename R>
-__global__ void k_step_final(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, R *__restrict__ xpos,
+__global__ void k_step_final(DevParams p, const Ctrl *__restrict__ ctrl, TileDev td, Ring<R> ring, R *__restrict__ xpos,
                              R *__restrict__ v, R *__restrict__ vl, const R *__restrict__ xt,
                              const int *__restrict__ mask) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= p.nn) return;
-  const R *__restrict__ x = ring.b[ctrl->cur];
+  const int inst = td.blk_inst[blockIdx.x];
+  const int n = td.blk_start[blockIdx.x] + (int)threadIdx.x;
+  if (n >= td.inst_node_off[inst + 1]) return;
+  const R *__restrict__ x = ring.b[ctrl[inst].cur];
   const R h = (R)p.h;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -649,13 +659,14 @@ void Launch<R>::setup_nodes(const DevMesh<R> &m, const DevParams &p, int bs, hip
   hipLaunchKernelGGL(k_setup_nodes, dim3(nblk(p.nn, bs)), dim3(bs), 0, st, p, m.inc_off, m.inc, m.vol, m.mask, m.mass, m.gforce);
 }
 template <typename R>
-void Launch<R>::tet(const DevMesh<R> &m, const DevParams &p, int bs, int force_hd, int write_e, hipStream_t st) {
+void Launch<R>::tet(const DevMesh<R> &m, const TileDev &td, const DevParams &p, int bs, int force_hd, int write_e,
+                    hipStream_t st) {
   if (p.nt <= 0) return;
   R *te = write_e ? m.tete : nullptr;
   if (p.material == 0)
-    hipLaunchKernelGGL((k_tet<R, 0>), dim3(nblk(p.nt, bs)), dim3(bs), 0, st, p, m.ctrl, m.ring, m.tets, m.dmi, m.vol, m.frec, m.hrec, m.etet, te, force_hd);
+    hipLaunchKernelGGL((k_tet<R, 0>), dim3(nblk(p.nt, bs)), dim3(bs), 0, st, p, m.ctrl, m.ring, m.tets, m.dmi, m.vol, m.frec, m.hrec, m.etet, te, force_hd, td);
   else
-    hipLaunchKernelGGL((k_tet<R, 1>), dim3(nblk(p.nt, bs)), dim3(bs), 0, st, p, m.ctrl, m.ring, m.tets, m.dmi, m.vol, m.frec, m.hrec, m.etet, te, force_hd);
+    hipLaunchKernelGGL((k_tet<R, 1>), dim3(nblk(p.nt, bs)), dim3(bs), 0, st, p, m.ctrl, m.ring, m.tets, m.dmi, m.vol, m.frec, m.hrec, m.etet, te, force_hd, td);
 }
 template <typename R>
 size_t Launch<R>::tile_lds(const TileDev &td) {
@@ -673,25 +684,27 @@ void Launch<R>::tile_pass(const DevMesh<R> &m, const TileDev &td, const DevParam
 }
 template <typename R>
 void Launch<R>::node_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
-  hipLaunchKernelGGL((k_node<R, false>), dim3(nblk(p.nn, 256)), dim3(256), 0, st, p, m.ctrl, m.ring, td, m.fown, m.hpart,
+  hipLaunchKernelGGL((k_node<R, false>), dim3(td.nblk), dim3(256), 0, st, p, m.ctrl, m.ring, td, m.fown, m.hpart,
                      m.fownh, m.hparth, m.hd, m.mass, m.mask, m.ext, m.xt);
 }
 template <typename R>
 void Launch<R>::grad_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
-  hipLaunchKernelGGL((k_node<R, true>), dim3(nblk(p.nn, 256)), dim3(256), 0, st, p, m.ctrl, m.ring, td, m.fown, m.hpart,
+  hipLaunchKernelGGL((k_node<R, true>), dim3(td.nblk), dim3(256), 0, st, p, m.ctrl, m.ring, td, m.fown, m.hpart,
                      m.fownh, m.hparth, m.hd, m.mass, m.mask, m.ext, m.xt);
 }
 template <typename R>
 void Launch<R>::control(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
-  hipLaunchKernelGGL(k_control, dim3(1), dim3(1024), 0, st, p, m.ctrl, td, nblk(p.nn, 256));
+  hipLaunchKernelGGL(k_control, dim3(td.ninst), dim3(1024), 0, st, p, m.ctrl, td);
 }
 template <typename R>
-void Launch<R>::step_init(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st) {
-  hipLaunchKernelGGL((k_step_init<R>), dim3(nblk(p.nn, bs)), dim3(bs), 0, st, p, m.ctrl, m.ring, m.xpos, m.v, m.vl, m.xt);
+void Launch<R>::step_init(const DevMesh<R> &m, const TileDev &td, const DevParams &p, int bs, hipStream_t st) {
+  hipLaunchKernelGGL((k_step_init<R>), dim3(nblk(std::max(p.nn, td.ninst), bs)), dim3(bs), 0, st, p, m.ctrl, td.ninst,
+                     m.ring, m.xpos, m.v, m.vl, m.xt);
 }
 template <typename R>
-void Launch<R>::step_final(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st) {
-  hipLaunchKernelGGL((k_step_final<R>), dim3(nblk(p.nn, bs)), dim3(bs), 0, st, p, m.ctrl, m.ring, m.xpos, m.v, m.vl, m.xt, m.mask);
+void Launch<R>::step_final(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
+  hipLaunchKernelGGL((k_step_final<R>), dim3(td.nblk), dim3(256), 0, st, p, m.ctrl, td, m.ring, m.xpos, m.v, m.vl, m.xt,
+                     m.mask);
 }
 template <typename R>
 void Launch<R>::standalone_ctrl(const DevMesh<R> &m, int cur, hipStream_t st) {

@@ -182,6 +182,7 @@ using namespace pbs;
 
 #define LEARN_GET(s, L)                                       \
   if (!(s)) return set_error(PBS_ERR_ARG, "null handle");     \
+  if ((s)->ninst != 1) return set_error(PBS_ERR_STATE, "the learned warm start needs a single-mesh solver"); \
   LearnState *L = nullptr;                                    \
   {                                                           \
     int rc_ = learn_get((s), &L);                             \
@@ -417,7 +418,7 @@ extern "C" int pbs_learn_features(pbs_solver *s, float *features) {
 // changed (the step-start kernel only writes the descent's scratch buffers).
 template <typename R> static int guess_impl(pbs_solver *s, LearnState *L, double *learned, double *plain) {
   DevMesh<R> &m = mesh<R>(s);
-  Launch<R>::step_init(m, s->dp, s->bs, s->st);
+  Launch<R>::step_init(m, s->td, s->dp, s->bs, s->st);
   LearnLaunch<R>::features(m, s->dp, L->adj_off, L->adj, L->feat, (R *)L->y, s->st);
   LearnLaunch<R>::guess(m, s->dp, mlp_dev(*L), L->feat, (const R *)L->y, s->st);
   LearnLaunch<R>::select(m, s->td, s->dp, L->adj_off, L->adj, L->smooth_iters, L->betas.data(), (int)L->betas.size(),

@@ -31,6 +31,11 @@ static const char *const kKernelNames[K_COUNT] = {"k_tile", "k_node", "k_control
 
 struct pbs_solver {
   int nn = 0, nt = 0;
+  int ninst = 1;                          // independent meshes (batched solver)
+  std::vector<float> inst_youngs, inst_nu;
+  std::vector<Ctrl> ctrl_h;               // last copy of the control blocks
+  std::vector<int> tune_tiles;            // autotuner: candidate tile sizes ...
+  std::vector<double> tune_us;            // ... and their time per descent pass
   pbs_params p{};
   DevParams dp{};
   int bs = 256, ntb = 0, nnb = 0;

@@ -198,12 +198,83 @@ bool build_tile_plan(int nn, const float *rest, int nt, const int *tets, int til
     t0 = t1;
   }
   P.ntiles = (int)P.tiles.size();
+  P.ninst = 1;
+  P.inst_node_off = {0, nn};
+  P.inst_tet_off = {0, nt};
+  P.inst_tile_off = {0, P.ntiles};
   P.halo_total = (long)P.halo.size();
   P.bnd_off.assign(nn + 1, 0);
   for (int n = 0; n < nn; ++n) P.bnd_off[n + 1] = P.bnd_off[n] + (int)bnd_lists[n].size();
   P.halo_slot.assign(P.halo.size(), 0);
   for (int n = 0; n < nn; ++n)
     for (size_t j = 0; j < bnd_lists[n].size(); ++j) P.halo_slot[bnd_lists[n][j]] = P.bnd_off[n] + (int)j;
+  return true;
+}
+
+bool build_batch_plan(int ninst, const int *node_off, const int *tet_off, const float *rest, const int *tets,
+                      int tile_tets, TilePlan &P, std::string &msg) {
+  P = TilePlan();
+  P.ninst = ninst;
+  P.nn = node_off[ninst];
+  P.nt = tet_off[ninst];
+  P.tile_tets = tile_tets;
+  P.inst_node_off.assign(node_off, node_off + ninst + 1);
+  P.inst_tet_off.assign(tet_off, tet_off + ninst + 1);
+  P.inst_tile_off.assign(1, 0);
+  P.bnd_off.assign(1, 0);
+  std::vector<int> lt;
+  for (int i = 0; i < ninst; ++i) {
+    const int n0 = node_off[i], n1 = node_off[i + 1], t0 = tet_off[i], t1 = tet_off[i + 1];
+    if (n1 < n0 || t1 < t0) {
+      msg = "instance offsets must not decrease";
+      return false;
+    }
+    lt.resize(4 * (size_t)(t1 - t0));
+    for (size_t k = 0; k < lt.size(); ++k) {
+      const int g = tets[4 * (size_t)t0 + k];
+      if (g < n0 || g >= n1) {
+        msg = "instance " + std::to_string(i) + ": tet node outside the instance's node range";
+        return false;
+      }
+      lt[k] = g - n0;
+    }
+    TilePlan Q;
+    if (!build_tile_plan(n1 - n0, rest + 3 * (size_t)n0, t1 - t0, lt.data(), tile_tets, Q, msg)) {
+      msg = "instance " + std::to_string(i) + ": " + msg;
+      return false;
+    }
+    const int tiles0 = (int)P.tiles.size(), off0 = (int)P.tile_off.size();
+    const long halo0 = (long)P.halo.size();
+    for (int v : Q.perm) P.perm.push_back(v + n0);
+    for (int v : Q.iperm) P.iperm.push_back(v + n0);
+    for (int v : Q.tperm) P.tperm.push_back(v + t0);
+    for (int v : Q.itperm) P.itperm.push_back(v + t0);
+    for (int v : Q.tets_new) P.tets_new.push_back(v + n0);
+    const int inc0 = P.inc_off.empty() ? 0 : P.inc_off.back();
+    if (P.inc_off.empty()) P.inc_off.push_back(0);
+    for (int n = 0; n < n1 - n0; ++n) P.inc_off.push_back(inc0 + Q.inc_off[n + 1]);
+    for (int v : Q.inc) P.inc.push_back(v + 4 * t0);
+    for (TileInfo ti : Q.tiles) {
+      ti.own_start += n0;
+      ti.halo_off += (int)halo0;
+      ti.tet_start += t0;
+      ti.off_start += off0;
+      ti.inst = i;
+      P.tiles.push_back(ti);
+    }
+    for (int v : Q.halo) P.halo.push_back(v + n0);
+    for (uint16_t v : Q.tile_off) P.tile_off.push_back(v);
+    for (uint32_t v : Q.tet_lp) P.tet_lp.push_back(v);
+    const int b0 = P.bnd_off.back();
+    for (int n = 0; n < n1 - n0; ++n) P.bnd_off.push_back(b0 + Q.bnd_off[n + 1]);
+    for (int v : Q.halo_slot) P.halo_slot.push_back(v + b0);
+    P.lcap = std::max(P.lcap, Q.lcap);
+    P.tcap = std::max(P.tcap, Q.tcap);
+    P.inst_tile_off.push_back(tiles0 + Q.ntiles);
+  }
+  if (P.inc_off.empty()) P.inc_off.push_back(0);
+  P.ntiles = (int)P.tiles.size();
+  P.halo_total = (long)P.halo.size();
   return true;
 }
 

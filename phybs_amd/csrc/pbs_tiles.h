@@ -25,11 +25,15 @@ struct TileInfo {
   int own_start, own_cnt;  // owned nodes
   int halo_off, halo_cnt;  // halo node ids: halo[halo_off .. +halo_cnt); partial slots likewise
   int tet_start, tet_cnt;  // owned tets (solver order)
-  int off_start, pad;      // run offsets: tile_off[off_start .. off_start + own_cnt + halo_cnt]
+  int off_start;           // run offsets: tile_off[off_start .. off_start + own_cnt + halo_cnt]
+  int inst;                // mesh instance (batched solvers; 0 otherwise)
 };
 
 struct TilePlan {
   int nn = 0, nt = 0;
+  // instances (independent meshes of a batched solver) own contiguous node, tet and tile ranges
+  int ninst = 1;
+  std::vector<int> inst_node_off, inst_tet_off, inst_tile_off;
   int tile_tets = 0, ntiles = 0;
   int lcap = 0;                     // max local nodes (owned + halo) over tiles
   int tcap = 0;                     // max tets over tiles
@@ -51,5 +55,10 @@ struct TilePlan {
 // Returns false (with msg) if a tile exceeds the 16-bit local index limits.
 bool build_tile_plan(int nn, const float *rest, int nt, const int *tets, int tile_tets, TilePlan &plan,
                      std::string &msg);
+// Batched: ninst meshes, instance i owning nodes [node_off[i], node_off[i+1]) and tets
+// [tet_off[i], tet_off[i+1]) of the concatenated arrays (tets hold global node ids inside their
+// instance's range).  Each instance is planned on its own; the plans are concatenated.
+bool build_batch_plan(int ninst, const int *node_off, const int *tet_off, const float *rest, const int *tets,
+                      int tile_tets, TilePlan &plan, std::string &msg);
 
 }  // namespace pbs

@@ -69,3 +69,24 @@ def test_handle_calls_reject_null_handle():
     assert L.pbs_reset(None) == P.ERR_ARG
     assert L.pbs_set_positions(None, None) == P.ERR_ARG
     L.pbs_destroy(None)
+
+
+def test_create_batch_rejects_bad_arguments_without_a_device():
+    p = P.default_params()
+    rest, tets = P.mesh_bar(1, 1, 1)
+    nodes = np.array([len(rest), len(rest)], np.int32)
+    ntet = np.array([len(tets), len(tets)], np.int32)
+    allrest = np.concatenate([rest, rest])
+    alltets = np.concatenate([tets, tets + len(rest)]).astype(np.int32)
+    E = np.array([1e5, 1e5], np.float32)
+    bad_nu = np.array([0.3, 0.5], np.float32)
+    h = C.c_void_p()
+    assert P.lib().pbs_create_batch(2, P._p(nodes), P._p(ntet), None, P._p(allrest), None, P._p(alltets), P._p(E),
+                                    P._p(bad_nu), C.byref(p), C.byref(h)) == P.ERR_ARG
+    cross = alltets.copy()
+    cross[0, 0] = len(rest)  # instance 0 tet pointing into instance 1
+    nu = np.array([0.3, 0.45], np.float32)
+    assert P.lib().pbs_create_batch(2, P._p(nodes), P._p(ntet), None, P._p(allrest), None, P._p(cross), P._p(E),
+                                    P._p(nu), C.byref(p), C.byref(h)) == P.ERR_MESH
+    assert P.lib().pbs_create_batch(0, P._p(nodes), P._p(ntet), None, P._p(allrest), None, P._p(alltets), P._p(E),
+                                    P._p(nu), C.byref(p), C.byref(h)) == P.ERR_ARG

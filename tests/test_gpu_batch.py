@@ -1,0 +1,98 @@
+"""GPU tests of the batched solver (independent meshes stepped together) and of the autotuner:
+a batch gives bitwise the results of separate solvers with the same tile size."""
+import sys
+
+import numpy as np
+import pytest
+
+import meshgen as M
+import phybs_amd as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def instances(seed=0, k=5):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(k):
+        nx, ny, nz = (int(v) for v in (rng.integers(2, 6), rng.integers(2, 6), rng.integers(4, 14)))
+        rest, tets = M.bar(nx, ny, nz, 0.05)
+        mask = M.clamp_x0(rest)
+        x0 = M.twisted_state(rest, nx, ny, nz, 0.05, float(rng.uniform(0, 90)), 0.01, 10 + i)
+        E = float(np.exp(rng.uniform(np.log(1e4), np.log(1e6))))
+        nu = float(rng.uniform(0.3, 0.49))
+        out.append((rest, tets, mask, x0, E, nu))
+    return out
+
+
+@pytest.mark.parametrize("mode", ["fixed_f32", "relgrad_f64"])
+def test_batch_equals_separate_solvers(mode):
+    inst = instances()
+    kw = dict(tile_tets=64, grav=(0.0, 0.0, -9.8))
+    if mode == "relgrad_f64":
+        kw.update(precision=P.FP64, relgrad_tol=1e-4, max_iterations=5000)
+    say("batch of", len(inst), mode, [len(t) for _, t, _, _, _, _ in inst])
+    sep_x, sep_st = [], []
+    for rest, tets, mask, x0, E, nu in inst:
+        s = P.Solver(rest, tets, mask, P.default_params(youngs=E, nu=nu, **kw))
+        s.set_positions(x0)
+        st = [s.step() for _ in range(4)]
+        sep_x.append(s.positions(f64=True))
+        sep_st.append([(q.iterations, q.rollbacks, q.passes) for q in st])
+        s.close()
+    b = P.Solver.batch([(r, t, m) for r, t, m, _, _, _ in inst], [i[4] for i in inst], [i[5] for i in inst],
+                       P.default_params(**kw))
+    b.set_positions(np.concatenate([i[3] for i in inst]))
+    per = [[] for _ in inst]
+    for _ in range(4):
+        agg = b.step()
+        for i in range(len(inst)):
+            q = b.instance_stats(i)
+            per[i].append((q.iterations, q.rollbacks, q.passes))
+        assert agg.passes == max(p[-1][2] for p in per)
+    xb = b.positions(f64=True)
+    b.close()
+    off = 0
+    for i, x in enumerate(sep_x):
+        assert per[i] == sep_st[i], (i, per[i], sep_st[i])
+        assert np.array_equal(xb[off:off + len(x)].view(np.uint64), x.view(np.uint64)), i
+        off += len(x)
+
+
+def test_batch_graph_replay_matches_eager():
+    inst = instances(seed=3, k=4)
+    res = []
+    for gp in (-1, 8):
+        b = P.Solver.batch([(r, t, m) for r, t, m, _, _, _ in inst], [i[4] for i in inst], [i[5] for i in inst],
+                           P.default_params(tile_tets=64, graph_passes=gp, alpha0=0.3))
+        b.set_positions(np.concatenate([i[3] for i in inst]))
+        for _ in range(3):
+            b.step()
+        res.append(b.positions())
+        b.close()
+    assert np.array_equal(res[0].view(np.uint32), res[1].view(np.uint32))
+
+
+def test_autotuner_picks_a_timed_tile_size():
+    rest, tets = M.bar(8, 8, 32, 0.05)
+    mask = M.clamp_x0(rest)
+    x0 = M.twisted_state(rest, 8, 8, 32, 0.05, 30.0, 0.01, 1)
+    s = P.Solver(rest, tets, mask, P.default_params(tile_tets=-1), nodes=x0)
+    rep = s.autotune_report()
+    p = s.get_params()
+    say("autotuner us/pass", rep, "chosen", p.tile_tets)
+    assert len(rep) >= 2 and p.tile_tets in rep and rep[p.tile_tets] == min(rep.values())
+    for _ in range(2):
+        s.step()
+    x_tuned = s.positions()
+    s.close()
+    t = P.Solver(rest, tets, mask, P.default_params(tile_tets=p.tile_tets), nodes=x0)
+    for _ in range(2):
+        t.step()
+    assert np.array_equal(t.positions().view(np.uint32), x_tuned.view(np.uint32))
+    t.close()

@@ -127,25 +127,47 @@ def test_rejected_guess_leaves_the_plain_trajectory_unchanged():
     assert np.array_equal(xp.view(np.uint64), xl.view(np.uint64))
 
 
-@pytest.mark.parametrize("net", ["zero", "random"])
+def trained_model(rest, tets, mask, gp, seed=5, steps=12):
+    """A network trained on another trajectory (seeded twist, impulses) of the same bar."""
+    t = P.Solver(rest, tets, mask, gp)
+    t.set_positions(M.twisted_state(rest, 4, 4, 16, 0.05, 60.0, 0.01, seed))
+    t.learn_collect(steps * len(rest))
+    rng = np.random.default_rng(seed)
+    for k in range(steps):
+        ext = np.zeros_like(rest)
+        if k % 4 == 2:
+            ext[rng.integers(0, len(rest), 10)] = rng.normal(0, 5.0, (10, 3))
+        t.step(ext)
+    t.learn_fit_scales()
+    t.learn_init_model(seed)
+    t.learn_train(40, batch=256, lr=3e-3, seed=seed)
+    m = t.learn_get_model()
+    t.close()
+    return m
+
+
+@pytest.mark.parametrize("net", ["trained", "random"])
 def test_warm_started_step_matches_oracle(net):
     """FP64: the GPU's learned guess handed to the oracle's warm-started step (which evaluates both
-    guesses' gradients and keeps the better) gives the same decision, iterations and positions.
-    The zero network guesses y = x + h v + h^2 g (accepted from rest), the random one is rejected."""
+    guesses' gradients and keeps the better) gives the same decision, iterations and positions,
+    for a trained network (its guess is taken in some steps) and a random one (always rejected)."""
     rest, tets, mask, x0 = bar()
     say("warm-started step vs oracle,", net, "network")
-    sc = SCALES.copy()
-    sc[6] = 2e-3
     gp = P.default_params(precision=P.FP64, relgrad_tol=1e-4, max_iterations=5000)
+    if net == "trained":
+        params, sc = trained_model(rest, tets, mask, gp)
+    else:
+        params, sc = model(), SCALES.copy()
+        sc[6] = 2e-3
     s = P.Solver(rest, tets, mask, gp)
-    s.set_positions(rest if net == "zero" else x0)
-    s.learn_set_model(np.zeros(P.MLP_PARAMS, np.float32) if net == "zero" else model(), sc)
+    s.set_positions(x0)
+    s.learn_set_model(params, sc)
     s.learn_enable(True)
     o = O.Solver(rest, tets, mask, params=O.default_params(relgrad_tol=1e-4, iterations=5000), precision=64)
-    o.set_state(x=(rest if net == "zero" else x0).astype(np.float64))
+    o.set_state(x=x0.astype(np.float64))
     h, mom = float(np.float32(0.03)), float(np.float32(0.2))
     used = []
-    for k in range(4):
+    for k in range(8):
         learned, plain = s.learn_guess()
         xs, vs, vls = o.state()
         assert np.abs(plain - (xs + h * vs + mom * (vs - vls) * h)).max() <= 1e-9
@@ -158,7 +180,8 @@ def test_warm_started_step_matches_oracle(net):
         assert abs(gs.iterations - os_.iterations) <= 2
         assert np.abs(s.positions(f64=True) - o.state()[0]).max() <= 1e-7 * 0.8
         used.append(gs.used_learned)
-    assert (net == "zero") == any(used)
+    say("guess used in steps", used)
+    assert (net == "trained") == any(used)
     s.close()
 
 
