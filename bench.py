@@ -30,7 +30,26 @@ WORKLOADS = {
     "c1": (12, 12, 50, 0, 90.0, 0.01, 96),
     "c2a": (64, 64, 256, 0, 0.0, 0.0, 64),
     "c2b": (64, 64, 256, 1, 0.0, 0.0, 64),
+    # 256 independent bars in one batched solver (sizes and materials from batch_instances)
+    "c3": (0, 0, 0, 0, 0.0, 0.0, 96),
 }
+
+
+def batch_instances(P, count=256, seed=3):
+    """Seeded bars nx, ny in [4, 16], nz in [8, 64], E log-uniform in [1e4, 1e6] Pa, nu uniform in
+    [0.3, 0.49], each clamped at x = 0 and starting at rest."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        nx, ny = int(rng.integers(4, 17)), int(rng.integers(4, 17))
+        nz = int(rng.integers(8, 65))
+        rest, tets = P.mesh_bar(nx, ny, nz, 0.05)
+        mask = (rest[:, 0] == 0).astype(np.int32)
+        x0 = rest
+        E = float(np.exp(rng.uniform(np.log(1e4), np.log(1e6))))
+        nu = float(rng.uniform(0.3, 0.49))
+        out.append((rest, tets, mask, x0, E, nu))
+    return out
 
 
 def algorithmic_bytes(nt, nn, rsz):
@@ -72,7 +91,31 @@ def cpu_baseline(rest, tets, mask, material, iters_per_step, grav, seconds=15.0)
             "iterations_per_s": its}
 
 
-WARM_OPTIONS = dict(sweeps=100, betas=(1.0, 0.5, 0.25, 0.1))
+def cpu_baseline_batch(inst, iters, grav, sample=8):
+    """The oracle (FP32, OpenMP) on the first `sample` instances, one step of `iters` iterations
+    each; the batch step time is extrapolated by tet count."""
+    from oracle import pyoracle as O
+
+    cores = min(16, len(os.sched_getaffinity(0)))
+    t_sum, tets_sum = 0.0, 0
+    for rest, tets, mask, x0, E, nu in inst[:sample]:
+        p = O.default_params(iterations=iters, grav=grav, dt=1.0 / 30.0, profile=[(1, 0.96), (7, 0.991), (10, 0.9995)])
+        o = O.Solver(rest, tets, mask, E=E, nu=nu, params=p, precision=32, threads=cores)
+        o.set_state(x=x0.astype(np.float64))
+        t0 = time.perf_counter()
+        o.step()
+        t_sum += time.perf_counter() - t0
+        tets_sum += len(tets)
+    total = sum(len(i[1]) for i in inst)
+    step_s = t_sum * total / tets_sum
+    return {"value": 1.0 / step_s, "unit": "steps/s", "cores": cores, "kind": "port",
+            "sample": f"{sample} of {len(inst)} instances, one {iters}-iteration FP32 step each in {t_sum:.1f} s "
+                      f"({tets_sum} tets), scaled by tet count to the whole batch"}
+
+
+# the learned correction is smoothed by 100 averaging sweeps and tried at 2, 1.5 and 1 times its
+# length (the best of the three, if better than the plain guess, starts the descent)
+WARM_OPTIONS = dict(sweeps=100, betas=(2.0, 1.5, 1.0))
 
 
 def impulse(rest, rng, k):
@@ -125,6 +168,7 @@ def timed_steps(P, rest, tets, mask, x0, params, warmup, steps, model=None):
     s.close()
     return {"steps_per_s": steps / dt, "iterations_per_step": float(np.mean([q.iterations for q in st])),
             "guess_used_fraction": float(np.mean([q.used_learned for q in st])),
+            "g_learned_over_g_plain": [q.g_learned / q.g0_norm for q in st] if model is not None else None,
             "converged_steps": int(sum(q.converged for q in st))}
 
 
@@ -132,7 +176,7 @@ def warm_start_report(P, args, params, rest, tets, mask, x0, dims, plain_sps, pl
     """Learned warm start vs plain: on the benchmark bar (trained on another trajectory of the same
     bar) and on the 12x12x50 bar (43,200 tets)."""
     out = {}
-    model, info = train_warm_start(P, dims, params, args.train_steps, 7, 8, 4_000_000)
+    model, info = train_warm_start(P, dims, params, args.train_steps, 7, 20, 8_000_000)
     lw = timed_steps(P, rest, tets, mask, x0, params, args.warmup, args.steps, model)
     out["bench_bar"] = {"training": info, "plain_steps_per_s": plain_sps, "plain_iterations_per_step": plain_its,
                         "learned": lw, "speedup": lw["steps_per_s"] / plain_sps}
@@ -162,7 +206,7 @@ def main():
     ap.add_argument("--max-iterations", type=int, default=40000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
-    ap.add_argument("--train-steps", type=int, default=3, help="training-trajectory steps on the benchmark bar")
+    ap.add_argument("--train-steps", type=int, default=6, help="training-trajectory steps on the benchmark bar")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) for counter calibration")
     args = ap.parse_args()
@@ -188,17 +232,28 @@ def main():
 
     nx, ny, nz, material, twist, jitter, fixed_iters = WORKLOADS[args.workload]
     edge = 0.05
-    rest, tets = P.mesh_bar(nx, ny, nz, edge)
-    mask = (rest[:, 0] == 0).astype(np.int32)
-    x0 = P.mesh_twist(rest, nx, ny, nz, edge, twist, jitter, 1) if (twist or jitter) else rest
+    batch = args.workload == "c3"
+    if batch:
+        inst = batch_instances(P)
+        rest = np.concatenate([i[0] for i in inst])
+        tets = np.concatenate([i[1] for i in inst])
+        mask = np.concatenate([i[2] for i in inst])
+        x0 = np.concatenate([i[3] for i in inst])
+    else:
+        rest, tets = P.mesh_bar(nx, ny, nz, edge)
+        mask = (rest[:, 0] == 0).astype(np.int32)
+        x0 = P.mesh_twist(rest, nx, ny, nz, edge, twist, jitter, 1) if (twist or jitter) else rest
     grav = (0.0, 0.0, -9.8)
     relg = args.mode == "relgrad"
     f64 = args.precision == "f64" or (args.precision == "auto" and relg)
     params = P.default_params(
         dt=1.0 / 30.0, grav=grav, material=material, precision=P.FP64 if f64 else P.FP32,
         iterations=fixed_iters, relgrad_tol=args.tol if relg else 0.0, max_iterations=args.max_iterations,
-        profile=[(1, 0.96), (7, 0.991), (10, 0.9995)])
-    s = P.Solver(rest, tets, mask, params)
+        profile=[(1, 0.96), (7, 0.991), (10, 0.9995)], tile_tets=-1 if batch else 0)
+    if batch:
+        s = P.Solver.batch([(i[0], i[1], i[2]) for i in inst], [i[4] for i in inst], [i[5] for i in inst], params)
+    else:
+        s = P.Solver(rest, tets, mask, params)
     s.set_positions(x0)
     for _ in range(args.warmup):
         s.step()
@@ -232,8 +287,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if f64 else "f32",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {nx}x{ny}x{nz} bar, {nt} tets, {nn} nodes, "
-                               f"{'stretch' if material else 'neo-Hookean'}, "
+        "config": {"workload": (f"c3: 256 bars (nx, ny 4-16, nz 8-64; E 1e4-1e6 Pa; nu 0.3-0.49) in one batched "
+                                f"solver, {nt} tets, {nn} nodes, " if batch else
+                                f"{args.workload}: {nx}x{ny}x{nz} bar, {nt} tets, {nn} nodes, ")
+                               + f"{'stretch' if material else 'neo-Hookean'}, "
                                + (f"rel-grad {args.tol:g}" if relg else f"{fixed_iters} iterations/step"),
                    "dt": 1.0 / 30.0, "tets": nt, "nodes": nn},
         "iterations_per_step": iters / args.steps,
@@ -247,11 +304,16 @@ def main():
                      "algorithmic_bytes_per_launch": alg[dom]},
         "learned_warm_start": None,
     }
+    if batch:
+        out["autotune_us_per_pass"] = s.autotune_report()
+        out["tile_tets"] = s.get_params().tile_tets
     s.close()
-    if relg and not args.no_learned:
+    if relg and not args.no_learned and not batch:
         out["learned_warm_start"] = warm_start_report(P, args, params, rest, tets, mask, x0, (nx, ny, nz),
                                                       steps_per_s, iters / args.steps)
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and batch:
+        out["cpu_baseline"] = cpu_baseline_batch(inst, fixed_iters, grav)
+    elif not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(rest, tets, mask, material, iters / args.steps, grav)
     print(json.dumps(out))
 
