@@ -251,6 +251,103 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblk) {
   return start + k;
 }
 
+// One tet of a tile: F from the tile's LDS positions (stride 4 R per local node), the material
+// response, and its four corner contributions written to their slots of the LDS array rec (W
+// values each): forces on phase 0 (plus the clamped energy on check passes), Hessian-diagonal
+// entries rounded to FP32 on phase 1.
+template <typename R, int MAT>
+__device__ __forceinline__ void tile_tet(int phase, const uint4 lp, const R Dmi[9], R V, const R *__restrict__ sx,
+                                         R *__restrict__ rec, R mu, R lam, int chk, int clamp, double &e_el) {
+  constexpr int W = ContribWidth<R>::value;
+  const R *p0 = sx + 4 * (lp.x & 0xffff), *p1 = sx + 4 * (lp.x >> 16), *p2 = sx + 4 * (lp.y & 0xffff),
+          *p3 = sx + 4 * (lp.y >> 16);
+  R Ds[9];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const R x0 = p0[k];
+    Ds[3 * k + 0] = p1[k] - x0;
+    Ds[3 * k + 1] = p2[k] - x0;
+    Ds[3 * k + 2] = p3[k] - x0;
+  }
+  R F[9];
+  mul3(Ds, Dmi, F);
+  R f12[12], h12[12], en;
+  const unsigned q[4] = {lp.z & 0xffff, lp.z >> 16, lp.w & 0xffff, lp.w >> 16};
+  if (phase == 0) {
+    if (MAT == 0) neo_tet<R, false>(F, Dmi, V, mu, lam, f12, en, h12);
+    else stretch_tet<R, false>(F, Dmi, V, mu, lam, f12, en, h12);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      R *r = rec + W * q[c];
+      r[0] = f12[3 * c];
+      r[1] = f12[3 * c + 1];
+      r[2] = f12[3 * c + 2];
+    }
+    if (chk) {
+      if (clamp == 1) en = en > (R)0 ? en : (R)0;          // CUDAInterface.cu:549
+      else if (clamp == 2) en = en > (R)1e-6 ? en : (R)0;  // IsotropicNeohookeanMaterial.cpp:129
+      e_el += (double)en;
+    }
+  } else {
+    if (MAT == 0) neo_tet<R, true>(F, Dmi, V, mu, lam, f12, en, h12);
+    else stretch_tet<R, true>(F, Dmi, V, mu, lam, f12, en, h12);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      R *r = rec + W * q[c];
+      r[0] = (R)(float)h12[3 * c];
+      r[1] = (R)(float)h12[3 * c + 1];
+      r[2] = (R)(float)h12[3 * c + 2];
+    }
+  }
+}
+
+// Gather of one tile: each thread sums the contribution runs of its local nodes in order and
+// writes one partial per owned node (3 planes of N) and per halo node (xyz at its slot).
+template <typename R>
+__device__ __forceinline__ void tile_gather(int phase, const TileInfo &ti, const TileDev &td, const R *__restrict__ rec,
+                                            int N, R *__restrict__ fown, R *__restrict__ hpart,
+                                            float *__restrict__ fownh, float *__restrict__ hparth) {
+  constexpr int W = ContribWidth<R>::value;
+  const int L = ti.own_cnt + ti.halo_cnt;
+  const unsigned short *off = td.tile_off + ti.off_start;
+  for (int i = threadIdx.x; i < L; i += blockDim.x) {
+    const int o0 = off[i], o1 = off[i + 1];
+    size_t d0, stride;
+    if (i < ti.own_cnt) {
+      d0 = (size_t)ti.own_start + i;
+      stride = N;
+    } else {
+      d0 = 3 * (size_t)td.halo_slot[ti.halo_off + (i - ti.own_cnt)];
+      stride = 1;
+    }
+    if (phase == 0) {
+      R a0 = (R)0, a1 = (R)0, a2 = (R)0;
+      for (int j = o0; j < o1; ++j) {
+        const R *r = rec + W * j;
+        a0 += r[0];
+        a1 += r[1];
+        a2 += r[2];
+      }
+      R *o = i < ti.own_cnt ? fown : hpart;
+      o[d0] = a0;
+      o[stride + d0] = a1;
+      o[2 * stride + d0] = a2;
+    } else {
+      float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+      for (int j = o0; j < o1; ++j) {
+        const R *r = rec + W * j;
+        a0 += (float)r[0];
+        a1 += (float)r[1];
+        a2 += (float)r[2];
+      }
+      float *o = i < ti.own_cnt ? fownh : hparth;
+      o[d0] = a0;
+      o[stride + d0] = a1;
+      o[2 * stride + d0] = a2;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Tile pass (per-tet half of computeForceKd, CUDAInterface.cu:495-568): one workgroup per tile
 // evaluates each of the tile's tets once -- force, energy on check passes, Hessian diagonal on
@@ -266,7 +363,6 @@ __global__ void __launch_bounds__(256)
 k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, const float *__restrict__ dmi,
        const float *__restrict__ vol, R *__restrict__ fown, R *__restrict__ hpart, float *__restrict__ fownh,
        float *__restrict__ hparth) {
-  constexpr int W = ContribWidth<R>::value;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double lds_red[16];
   const int b = xcd_tile(blockIdx.x, gridDim.x);
@@ -294,87 +390,10 @@ k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
       R Dmi[9];
 #pragma unroll
       for (int k = 0; k < 9; ++k) Dmi[k] = (R)dmi[k * (size_t)T + t];
-      const R V = (R)vol[t];
-      const R *p0 = sx + 4 * (lp.x & 0xffff), *p1 = sx + 4 * (lp.x >> 16), *p2 = sx + 4 * (lp.y & 0xffff),
-              *p3 = sx + 4 * (lp.y >> 16);
-      R Ds[9];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const R x0 = p0[k];
-        Ds[3 * k + 0] = p1[k] - x0;
-        Ds[3 * k + 1] = p2[k] - x0;
-        Ds[3 * k + 2] = p3[k] - x0;
-      }
-      R F[9];
-      mul3(Ds, Dmi, F);
-      R f12[12], h12[12], en;
-      const unsigned q[4] = {lp.z & 0xffff, lp.z >> 16, lp.w & 0xffff, lp.w >> 16};
-      if (phase == 0) {
-        if (MAT == 0) neo_tet<R, false>(F, Dmi, V, mu, lam, f12, en, h12);
-        else stretch_tet<R, false>(F, Dmi, V, mu, lam, f12, en, h12);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          R *r = rec + W * q[c];
-          r[0] = f12[3 * c];
-          r[1] = f12[3 * c + 1];
-          r[2] = f12[3 * c + 2];
-        }
-        if (chk) {
-          if (p.clamp == 1) en = en > (R)0 ? en : (R)0;          // CUDAInterface.cu:549
-          else if (p.clamp == 2) en = en > (R)1e-6 ? en : (R)0;  // IsotropicNeohookeanMaterial.cpp:129
-          e_el += (double)en;
-        }
-      } else {
-        if (MAT == 0) neo_tet<R, true>(F, Dmi, V, mu, lam, f12, en, h12);
-        else stretch_tet<R, true>(F, Dmi, V, mu, lam, f12, en, h12);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          R *r = rec + W * q[c];
-          r[0] = (R)(float)h12[3 * c];
-          r[1] = (R)(float)h12[3 * c + 1];
-          r[2] = (R)(float)h12[3 * c + 2];
-        }
-      }
+      tile_tet<R, MAT>(phase, lp, Dmi, (R)vol[t], sx, rec, mu, lam, chk, p.clamp, e_el);
     }
     __syncthreads();
-    const unsigned short *off = td.tile_off + ti.off_start;
-    for (int i = tid; i < L; i += 256) {
-      const int o0 = off[i], o1 = off[i + 1];
-      // owned partials: 3 planes of N; halo partials: 3 consecutive values per slot
-      size_t d0, stride;
-      if (i < ti.own_cnt) {
-        d0 = (size_t)ti.own_start + i;
-        stride = N;
-      } else {
-        d0 = 3 * (size_t)td.halo_slot[ti.halo_off + (i - ti.own_cnt)];
-        stride = 1;
-      }
-      if (phase == 0) {
-        R a0 = (R)0, a1 = (R)0, a2 = (R)0;
-        for (int j = o0; j < o1; ++j) {
-          const R *r = rec + W * j;
-          a0 += r[0];
-          a1 += r[1];
-          a2 += r[2];
-        }
-        R *o = i < ti.own_cnt ? fown : hpart;
-        o[d0] = a0;
-        o[stride + d0] = a1;
-        o[2 * stride + d0] = a2;
-      } else {
-        float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-        for (int j = o0; j < o1; ++j) {
-          const R *r = rec + W * j;
-          a0 += (float)r[0];
-          a1 += (float)r[1];
-          a2 += (float)r[2];
-        }
-        float *o = i < ti.own_cnt ? fownh : hparth;
-        o[d0] = a0;
-        o[stride + d0] = a1;
-        o[2 * stride + d0] = a2;
-      }
-    }
+    tile_gather<R>(phase, ti, td, rec, N, fown, hpart, fownh, hparth);
     if (phase == 0 && hdnow) __syncthreads();  // the Hessian phase reuses the contribution slots
   }
   if (chk) {
