@@ -200,6 +200,21 @@ int64_t pbs_csr_pattern(int num_nodes, int num_tets, const int *tets, int *rowpt
  * ProjDynamic.cpp:25-45): offsets N+1, entries 4T = 12*tet + 3*local. */
 int pbs_incidence(int num_nodes, int num_tets, const int *tets, int *offsets, int *entries);
 
+/* ---- interactive external forces (host; the runtime input of doDescentOpt) ---- */
+/* Ring of each boundary face (TetMesh::computeBoundfaceRingIndices, TetMesh.cpp:229-276): its 3
+ * nodes in face order, then the other nodes sharing a boundary edge with any of them, ascending.
+ * Returns the total length; fills offsets (F+1) and ring when non-NULL. */
+int64_t pbs_face_ring(int num_nodes, int num_faces, const int *faces, int64_t *offsets, int *ring);
+/* Face hit by a ray (TetMesh::pickFacebyRay, TetMesh.cpp:393-418): *face = the face facing the ray
+ * whose centre is nearest to it, within the length of face 0's first edge; -1 if none. */
+int pbs_pick_face(int num_nodes, const float *nodes, int num_faces, const int *faces, const float *origin,
+                  const float *direction, int *face);
+/* External forces of dragging a face (TetMesh::dragFaceRing, TetMesh.cpp:446-470): factor =
+ * min(|drag| / |p1 - p0 of face 0| * 100, 800); factor * mass * drag direction on the face's nodes,
+ * half of it on the rest of its ring; ext (3N AoS) is overwritten. */
+int pbs_drag_face_ring(int num_nodes, const float *nodes, const float *mass, int num_faces, const int *faces,
+                       const int64_t *ring_offsets, const int *ring, int face, const float *drag, float *ext);
+
 /* ---- learned warm start (no counterpart in the reference, which has no learned component) ----
  * A per-vertex network (18 features -> 64 -> 64 -> 64 -> 3, SiLU; BF16 weights on the matrix
  * cores, FP32 accumulation) predicts the displacement of the converged step from the inertial

@@ -87,6 +87,9 @@ SIGNATURES = {
     "pbs_node_read": (C.c_int, [C.c_char_p, C.c_int, _P]),
     "pbs_csr_pattern": (C.c_int64, [C.c_int, C.c_int, _P, _P, _P, _P, _P]),
     "pbs_incidence": (C.c_int, [C.c_int, C.c_int, _P, _P, _P]),
+    "pbs_face_ring": (C.c_int64, [C.c_int, C.c_int, _P, _P, _P]),
+    "pbs_pick_face": (C.c_int, [C.c_int, _P, C.c_int, _P, _P, _P, _P]),
+    "pbs_drag_face_ring": (C.c_int, [C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_int, _P, _P]),
     "pbs_learn_set_model": (C.c_int, [_P, _P, _P]),
     "pbs_learn_get_model": (C.c_int, [_P, _P, _P]),
     "pbs_learn_init_model": (C.c_int, [_P, C.c_uint64]),
@@ -157,6 +160,35 @@ def _f32(a, n=None):
 
 
 # ---- host mesh utilities ----
+def face_ring(nn, faces):
+    faces = np.ascontiguousarray(faces, np.int32).reshape(-1, 3)
+    total = _check(lib().pbs_face_ring(nn, len(faces), _p(faces), None, None))
+    off = np.zeros(len(faces) + 1, np.int64)
+    ring = np.zeros(total, np.int32)
+    _check(lib().pbs_face_ring(nn, len(faces), _p(faces), _p(off), _p(ring)))
+    return off, ring
+
+
+def pick_face(nodes, faces, origin, direction):
+    nodes = _f32(nodes, (-1, 3))
+    faces = np.ascontiguousarray(faces, np.int32).reshape(-1, 3)
+    o = _f32(origin)
+    d = _f32(direction)
+    out = C.c_int(-1)
+    _check(lib().pbs_pick_face(len(nodes), _p(nodes), len(faces), _p(faces), _p(o), _p(d), C.byref(out)))
+    return out.value
+
+
+def drag_face_ring(nodes, mass, faces, ring_offsets, ring, face, drag):
+    nodes = _f32(nodes, (-1, 3))
+    mass = _f32(mass)
+    faces = np.ascontiguousarray(faces, np.int32).reshape(-1, 3)
+    off = np.ascontiguousarray(ring_offsets, np.int64)
+    ring = np.ascontiguousarray(ring, np.int32)
+    ext = np.zeros_like(nodes)
+    _check(lib().pbs_drag_face_ring(len(nodes), _p(nodes), _p(mass), len(faces), _p(faces), _p(off), _p(ring), face,
+                                    _p(_f32(drag)), _p(ext)))
+    return ext
 def mesh_bar(nx, ny, nz, edge=0.05):
     nn = (nx + 1) * (ny + 1) * (nz + 1)
     nodes = np.zeros((nn, 3), np.float32)

@@ -250,3 +250,110 @@ extern "C" int64_t pbs_csr_pattern(int nn, int nt, const int *tets, int *rowptr,
         }
   return nnz;
 }
+
+// ---- interactive external forces (TetMesh.cpp:229-276, 378-470; RenderWidget.cpp:150-195) ----
+
+// Face ring of each boundary face: its 3 nodes in face order, then the other nodes adjacent to
+// any of them along boundary-face edges, ascending (computeBoundfaceRingIndices).
+extern "C" int64_t pbs_face_ring(int nn, int nf, const int *faces, int64_t *offsets, int *ring) {
+  if (nn <= 0 || nf < 0 || (nf > 0 && !faces)) return set_error(PBS_ERR_ARG, "pbs_face_ring: bad arguments");
+  for (long i = 0; i < 3L * nf; ++i)
+    if (faces[i] < 0 || faces[i] >= nn) return set_error(PBS_ERR_MESH, "face index out of range");
+  std::vector<std::vector<int>> nb(nn);
+  for (int f = 0; f < nf; ++f)
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        if (a != b) nb[faces[3 * f + a]].push_back(faces[3 * f + b]);
+  for (auto &v : nb) {
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  }
+  int64_t total = 0;
+  std::vector<int> u;
+  for (int f = 0; f < nf; ++f) {
+    const int *q = faces + 3 * f;
+    u.clear();
+    for (int a = 0; a < 3; ++a) u.insert(u.end(), nb[q[a]].begin(), nb[q[a]].end());
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    u.erase(std::remove_if(u.begin(), u.end(), [&](int n) { return n == q[0] || n == q[1] || n == q[2]; }), u.end());
+    if (offsets) offsets[f] = total;
+    if (ring) {
+      for (int a = 0; a < 3; ++a) ring[total + a] = q[a];
+      for (size_t k = 0; k < u.size(); ++k) ring[total + 3 + (int64_t)k] = u[k];
+    }
+    total += 3 + (int64_t)u.size();
+  }
+  if (offsets) offsets[nf] = total;
+  return total;
+}
+
+// Face picked by a ray (pickFacebyRay): among faces whose outward normal (v2 x v1, v1 = p1 - p0,
+// v2 = p2 - p0) points against the direction, the one whose centre has the smallest squared
+// distance to the ray, if below the squared length of face 0's first edge; -1 if none.
+extern "C" int pbs_pick_face(int nn, const float *nodes, int nf, const int *faces, const float *orig,
+                             const float *dir, int *face) {
+  if (nn <= 0 || !nodes || nf <= 0 || !faces || !orig || !dir || !face)
+    return set_error(PBS_ERR_ARG, "pbs_pick_face: bad arguments");
+  for (long i = 0; i < 3L * nf; ++i)
+    if (faces[i] < 0 || faces[i] >= nn) return set_error(PBS_ERR_MESH, "face index out of range");
+  auto P = [&](int n, int d) { return nodes[3 * (size_t)n + d]; };
+  float e[3];
+  for (int d = 0; d < 3; ++d) e[d] = P(faces[1], d) - P(faces[0], d);
+  const float r = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+  const float r2 = r * r;
+  float best = 1e8f;
+  int pick = -1;
+  for (int f = 0; f < nf; ++f) {
+    const int *q = faces + 3 * f;
+    float v1[3], v2[3], c[3];
+    for (int d = 0; d < 3; ++d) {
+      v1[d] = P(q[1], d) - P(q[0], d);
+      v2[d] = P(q[2], d) - P(q[0], d);
+      c[d] = (P(q[0], d) + P(q[1], d) + P(q[2], d)) / 3.0f;
+    }
+    float nrm[3] = {v2[1] * v1[2] - v2[2] * v1[1], v2[2] * v1[0] - v2[0] * v1[2], v2[0] * v1[1] - v2[1] * v1[0]};
+    const float len = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    if (len > 0)
+      for (float &x : nrm) x /= len;
+    if (nrm[0] * dir[0] + nrm[1] * dir[1] + nrm[2] * dir[2] >= 0) continue;
+    float ov[3];
+    for (int d = 0; d < 3; ++d) ov[d] = orig[d] - c[d];
+    const float a = ov[0] * ov[0] + ov[1] * ov[1] + ov[2] * ov[2];
+    const float b = ov[0] * dir[0] + ov[1] * dir[1] + ov[2] * dir[2];
+    const float d2 = a - b * b;
+    if (d2 < r2 && d2 < best) {
+      pick = f;
+      best = d2;
+    }
+  }
+  *face = pick;
+  return PBS_OK;
+}
+
+// External forces of dragging a picked face (dragFaceRing): factor = min(|drag| / r * 100, 800)
+// with r = |p1 - p0| of face 0; ext = 0, then += factor * mass * drag/|drag| on the face's 3
+// nodes and half of that on the rest of its ring.  ext: 3N AoS floats.
+extern "C" int pbs_drag_face_ring(int nn, const float *nodes, const float *mass, int nf, const int *faces,
+                                  const int64_t *ring_offsets, const int *ring, int face, const float *drag,
+                                  float *ext) {
+  if (nn <= 0 || !nodes || !mass || nf <= 0 || !faces || !ring_offsets || !ring || !drag || !ext)
+    return set_error(PBS_ERR_ARG, "pbs_drag_face_ring: bad arguments");
+  if (face < 0 || face >= nf) return set_error(PBS_ERR_ARG, "face out of range");
+  std::fill(ext, ext + 3 * (size_t)nn, 0.0f);
+  float e[3];
+  for (int d = 0; d < 3; ++d) e[d] = nodes[3 * (size_t)faces[1] + d] - nodes[3 * (size_t)faces[0] + d];
+  const float r = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+  const float dl = std::sqrt(drag[0] * drag[0] + drag[1] * drag[1] + drag[2] * drag[2]);
+  if (!(r > 0) || !(dl > 0)) return PBS_OK;
+  float factor = dl / r * 100.0f;
+  if (factor > 800.0f) factor = 800.0f;
+  const float dir[3] = {drag[0] / dl, drag[1] / dl, drag[2] / dl};
+  for (int64_t k = ring_offsets[face]; k < ring_offsets[face + 1]; ++k) {
+    const int n = ring[k];
+    if (n < 0 || n >= nn) return set_error(PBS_ERR_MESH, "ring index out of range");
+    const float w = (k - ring_offsets[face] < 3 ? 1.0f : 0.5f) * factor * mass[n];
+    for (int d = 0; d < 3; ++d) ext[3 * (size_t)n + d] += w * dir[d];
+  }
+  return PBS_OK;
+}

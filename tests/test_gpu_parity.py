@@ -307,6 +307,35 @@ def test_external_force_matches_oracle():
     s.close()
 
 
+def test_dragged_face_ring_matches_oracle():
+    """The interactive input path on a TetGen mesh: a ray picks a boundary face, dragging it sets
+    the ring's external forces (host C ABI), and the GPU steps under them like the oracle."""
+    nodes, tets, faces = P.tetgen_load(os.path.join(M.GOLDEN, "model_bar/bar.1"), 0.1, faces=True)
+    mask = (nodes[:, 0] == nodes[:, 0].min()).astype(np.int32)
+    off, ring = P.face_ring(len(nodes), faces)
+    far = int(np.argmax(nodes[faces].mean(axis=1)[:, 0]))  # a face at the free end
+    f = faces[far]
+    n = np.cross(nodes[f[2]] - nodes[f[0]], nodes[f[1]] - nodes[f[0]])
+    n /= np.linalg.norm(n)
+    c = nodes[f].mean(axis=0)
+    pick = P.pick_face(nodes, faces, c + 0.3 * n, -n)
+    say("picked face", pick, "of", len(faces), "ring", off[pick + 1] - off[pick])
+    assert pick >= 0
+    mass = oracle_pre(nodes, tets, mask)["mass"]
+    ext = P.drag_face_ring(nodes, mass, faces, off, ring, pick, np.float32([0.0, 0.02, 0.01]))
+    assert np.count_nonzero(np.abs(ext).sum(axis=1)) == off[pick + 1] - off[pick]
+    s = P.Solver(nodes, tets, mask, P.default_params())
+    o = O.Solver(nodes, tets, mask, precision=32)
+    for _ in range(10):
+        s.step(ext)
+        o.step(ext.astype(np.float64))
+    L = float(nodes[:, 0].max() - nodes[:, 0].min())
+    err = np.abs(s.positions() - o.state()[0]).max()
+    say("max |x_gpu - x_oracle| / L", err / L)
+    assert err <= 1e-4 * L
+    s.close()
+
+
 def test_inverted_rest_tet_is_rejected():
     rest, tets, mask, _ = bar_case(1, 1, 1, twist=0.0, jitter=0.0)
     bad = tets.copy()

@@ -138,3 +138,43 @@ def test_adjacency_rejects_out_of_range():
     with pytest.raises(P.PhybsError) as e:
         P.incidence(3, tets)
     assert e.value.code == P.ERR_MESH
+
+
+def faces_fixture():
+    nodes, tets, faces = P.tetgen_load(os.path.join(M.GOLDEN, "model_bar/bar.1"), 1.0, faces=True)
+    return nodes, tets, faces
+
+
+def test_face_ring_brute_force():
+    nodes, _, faces = faces_fixture()
+    off, ring = P.face_ring(len(nodes), faces)
+    nb = [set() for _ in range(len(nodes))]
+    for f in faces:
+        for a in f:
+            nb[a].update(int(b) for b in f if b != a)
+    for i, f in enumerate(faces):
+        want = list(f) + sorted(set().union(*(nb[a] for a in f)) - set(int(a) for a in f))
+        assert list(ring[off[i]:off[i + 1]]) == want
+
+
+def test_pick_face_and_drag_forces():
+    nodes, _, faces = faces_fixture()
+    off, ring = P.face_ring(len(nodes), faces)
+    # a ray straight at the centre of face 7, from outside along its inward normal
+    f = faces[7]
+    p0, p1, p2 = nodes[f[0]], nodes[f[1]], nodes[f[2]]
+    n = np.cross(p2 - p0, p1 - p0)
+    n /= np.linalg.norm(n)
+    c = (p0 + p1 + p2) / 3
+    assert P.pick_face(nodes, faces, c + 5 * n, -n) == 7
+    assert P.pick_face(nodes, faces, c + 5 * n + 1e3, -n) == -1  # passes far from every face
+    mass = np.linspace(1.0, 2.0, len(nodes)).astype(np.float32)
+    drag = np.array([0.0, 0.3, 0.4], np.float32)
+    ext = P.drag_face_ring(nodes, mass, faces, off, ring, 7, drag)
+    r = np.linalg.norm(nodes[faces[0][1]] - nodes[faces[0][0]])
+    factor = min(0.5 / r * 100, 800)
+    want = np.zeros_like(nodes)
+    for k, nd in enumerate(ring[off[7]:off[8]]):
+        want[nd] += (1.0 if k < 3 else 0.5) * factor * mass[nd] * drag / 0.5
+    np.testing.assert_allclose(ext, want, rtol=1e-5)
+    assert not ext[np.setdiff1d(np.arange(len(nodes)), ring[off[7]:off[8]])].any()
