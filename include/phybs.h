@@ -176,6 +176,34 @@ int pbs_reset_kernel_times(pbs_solver *s);
  * the loop has stopped within a replay are not counted. */
 int pbs_set_profiling(pbs_solver *s, int period);
 
+/* ---- backward Euler (the reference's GPU implicit path, fvm.cpp:179-183) ---- */
+typedef struct pbs_implicit_params {
+  float damping_alpha;   /* 0.02: mass damping, M (1 + alpha h) (m_dumpingAlpha, CUDAInterface.cu:644) */
+  float damping_beta;    /* 0.02: stiffness damping, K (h + beta) h (m_dumpingBelta) */
+  double cg_tolerance;   /* 1e-6: stop at |D^-1/2 (b - A v)| <= tol |D^-1/2 b|, D = diag(A) (FP64) */
+  int cg_max_iterations; /* 10000 (the cap of CUDALinearSolvers::conjugateGradient) */
+  int warm_start;        /* 1: CG starts from the current velocities (as that solver does), 0: zero */
+} pbs_implicit_params;
+typedef struct pbs_implicit_stats {
+  int cg_iterations;
+  int converged;         /* 0: stopped at cg_max_iterations (the approximate solution is applied) */
+  double residual0, residual, rhs_norm; /* the Jacobi-scaled norms of b - A v0, b - A v, b */
+  float gpu_ms;          /* device time of the step */
+} pbs_implicit_stats;
+int pbs_implicit_default_params(pbs_implicit_params *p);
+/* One backward-Euler step (CUDAInterface::computeInnerforces + computeGlobalStiffnessMatrix +
+ * doBackEuler, CUDAInterface.cu:784-884): with f the elastic forces plus gravity (fixed nodes: the
+ * spring 1e8 (rest - x) instead) and ext (3N AoS, may be NULL: the reference has none),
+ *   [(1 + alpha h) M + (h + beta) h K(x)] v' = h f + M v,   x' = x + h v'   (all nodes),
+ * solved by Jacobi-preconditioned CG on the device (the reference factorises with cuSOLVER's
+ * sparse Cholesky).  out (3N AoS, may be NULL) receives x'.  A system that is not positive
+ * definite leaves the state unchanged and returns PBS_ERR_STATE.  Not available on batches. */
+int pbs_implicit_step(pbs_solver *s, const pbs_implicit_params *p, const float *ext, float *out,
+                      pbs_implicit_stats *st);
+/* K(x) = -df/dx at the current positions in the reference's CSR layout (computeGlobalK,
+ * CUDAInterface.cu:234-271; pattern = pbs_csr_pattern's rowptr/col, nnz values). */
+int pbs_stiffness_matrix(pbs_solver *s, int64_t nnz, double *csr_val);
+
 /* ---- host mesh utilities (no GPU work) ---- */
 /* Bar of nx*ny*nz cubes of size edge, 6 tets per cube (Freudenthal split, det(Dm) > 0).
  * nodes: 3*(nx+1)(ny+1)(nz+1) floats, tets: 24*nx*ny*nz ints; node id = i + (nx+1)(j + (ny+1)k). */

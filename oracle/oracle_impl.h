@@ -315,6 +315,103 @@ void FN(orc_hessian_diag)(int nn, int nt, const int *tets, const R *x, const flo
   free(per);
 }
 
+/* Full per-tet stiffness K = -df/dx (12 x 12, row 3n+d, column 3m+e), by the reference's chain
+ * (compute_dfdx, CUDAInterface.cu:178-219; IsotropicMaterial.cpp:503-552): with M = dP/dF from
+ * orc_dpdf, an_n the AN column of node n and c_m the Dm^-1 row of node m (node 0: minus sums),
+ *   K(n,d),(m,e) = -sum_{b,c} an_n[b] dP_(d,b)/dF_(e,c) c_m[c]. */
+static void FN(tet_stiffness)(const R *F, const float *dmi, const float *an, float mu, float lam, int material,
+                              R *K) {
+  R M[81];
+  FN(orc_dpdf)(F, mu, lam, material, M);
+  R anv[4][3], cv[4][3];
+  for (int n = 0; n < 4; ++n)
+    for (int c = 0; c < 3; ++c) {
+      if (n == 0) {
+        anv[n][c] = -((R)an[3 * c] + (R)an[3 * c + 1] + (R)an[3 * c + 2]);
+        cv[n][c] = -((R)dmi[c] + (R)dmi[3 + c] + (R)dmi[6 + c]);
+      } else {
+        anv[n][c] = (R)an[3 * c + n - 1];
+        cv[n][c] = (R)dmi[3 * (n - 1) + c];
+      }
+    }
+  for (int n = 0; n < 4; ++n)
+    for (int d = 0; d < 3; ++d)
+      for (int m = 0; m < 4; ++m)
+        for (int e = 0; e < 3; ++e) {
+          R acc = 0;
+          for (int b = 0; b < 3; ++b) {
+            R inner = 0;
+            for (int c = 0; c < 3; ++c) inner += M[(3 * d + b) * 9 + 3 * e + c] * cv[m][c];
+            acc += anv[n][b] * inner;
+          }
+          K[(3 * n + d) * 12 + 3 * m + e] = -acc;
+        }
+}
+
+/* Backward-Euler linear system of the reference's GPU implicit path (fvm.cpp:179-183):
+ *   computeElasticForces (CUDAInterface.cu:345-416): P = U diag(dpsi/ds) V^T from the SVD,
+ *     forces = P AN scattered as in tet_force;
+ *   addFixContraintsAndGravity (:327-343): fixed nodes f = 1e8 (rest - x), free nodes f += m g;
+ *   computeGlobalK (:234-271): kval[kID[48t + 4(3 fi + fj) + ni] + nj] += K(fi,fj),(ni,nj);
+ *   doBackEuler (:819-852): b = h f + m v (setRHSofLinearSystem), A = K (h + beta) h
+ *     (scaleDeviceArray), A_ii += m (1 + alpha h) (setMassScaled + setLHSofLinearSystem).
+ * ext (3N, may be NULL) is added to the free nodes' forces (the reference has no such input).
+ * Sums run in ascending tet order in precision R; outputs are widened to double. */
+void FN(orc_implicit_system)(int nn, const float *rest, int nt, const int *tets, const int *mask,
+                             const float *dminv, const float *an, const float *mass, const float *gforce,
+                             float mu_f, float lam_f, int material, const R *x, const R *v, const R *ext,
+                             float h_f, float alpha_f, float beta_f, long nnz, const int *kid, const int *diag_idx,
+                             double *kval, double *aval, double *b) {
+  const R mu = (R)mu_f, lam = (R)lam_f, h = (R)h_f;
+  R *kv = (R *)calloc((size_t)(nnz > 0 ? nnz : 1), sizeof(R));
+  R *f = (R *)calloc(3 * (size_t)nn, sizeof(R));
+  for (int t = 0; t < nt; ++t) {
+    const int *q = tets + 4 * t;
+    R F[9], U[9], s[3], V[9], ph[3], H[9], P[9], AN[9], G[9], K[144];
+    FN(def_grad)(q, x, dminv + 9 * t, F);
+    FN(orc_svd3)(F, U, s, V);
+    FN(fhat_grad_hess)(s, mu, lam, material, ph, H);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c)
+        P[3 * r + c] = U[3 * r] * ph[0] * V[3 * c] + U[3 * r + 1] * ph[1] * V[3 * c + 1] + U[3 * r + 2] * ph[2] * V[3 * c + 2];
+    for (int i = 0; i < 9; ++i) AN[i] = (R)an[9 * t + i];
+    FN(mul3)(P, AN, G);
+    for (int a = 0; a < 3; ++a) {
+      f[3 * q[1] + a] += G[3 * a + 0];
+      f[3 * q[2] + a] += G[3 * a + 1];
+      f[3 * q[3] + a] += G[3 * a + 2];
+      f[3 * q[0] + a] += -(G[3 * a + 0] + G[3 * a + 1] + G[3 * a + 2]);
+    }
+    FN(tet_stiffness)(F, dminv + 9 * t, an + 9 * t, mu_f, lam_f, material, K);
+    for (int fi = 0; fi < 4; ++fi)
+      for (int fj = 0; fj < 3; ++fj)
+        for (int ni = 0; ni < 4; ++ni)
+          for (int nj = 0; nj < 3; ++nj)
+            kv[kid[48 * (size_t)t + 4 * (3 * fi + fj) + ni] + nj] += K[(3 * fi + fj) * 12 + 3 * ni + nj];
+  }
+  const R s1 = h + (R)beta_f;
+  for (long i = 0; i < nnz; ++i) {
+    kval[i] = (double)kv[i];
+    aval[i] = (double)(kv[i] * s1 * h);
+  }
+  const R ms = (R)1 + (R)alpha_f * h;
+  for (int n = 0; n < nn; ++n)
+    for (int d = 0; d < 3; ++d) {
+      const long di = diag_idx[3 * n + d];
+      aval[di] = (double)((R)aval[di] + (R)mass[n] * ms);
+      R fk;
+      if (mask[n]) {
+        fk = (R)(1e8 * (double)((R)rest[3 * n + d] - x[3 * n + d]));
+      } else {
+        fk = f[3 * n + d] + (R)gforce[3 * n + d];
+        if (ext) fk += ext[3 * n + d];
+      }
+      b[3 * n + d] = (double)(fk * h + (R)mass[n] * v[3 * n + d]);
+    }
+  free(kv);
+  free(f);
+}
+
 /* ------------------------------------------------------------------------------------------
  * Descent solver state in precision R.
  * ------------------------------------------------------------------------------------------ */

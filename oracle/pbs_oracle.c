@@ -21,6 +21,10 @@
  *                       index form in numpy (test_oracle.py::test_dpdf_*)
  *   orc_hessian_diag    diagonal of -df/dx by central differences of the forces
  *                       (test_oracle.py::test_hessian_diag_fd)
+ *   orc_implicit_system K = -df/dx by central differences of its own right-hand side (whose
+ *                       forces come from the SVD route, pinned against the closed-form forces),
+ *                       translation null space K t = 0, symmetry, the system matrix against
+ *                       its definition (test_oracle.py::test_implicit_*)
  *   orc_csr_pattern     brute force from the tet list: row pattern, diagonal index, kID
  *                       (test_mesh.py::test_csr_pattern_bit_exact)
  *   orc_incidence       brute force (test_mesh.py::test_incidence_bit_exact)

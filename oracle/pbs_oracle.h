@@ -111,6 +111,22 @@ void orc_hessian_diag_f32(int nn, int nt, const int *tets, const float *x, const
 void orc_hessian_diag_f64(int nn, int nt, const int *tets, const double *x, const float *dminv,
                           const float *an, float mu, float lam, int material, double *hd, double *tethd);
 
+/* ---- backward-Euler linear system (fvm.cpp:179-183; CUDAInterface.cu:234-271, 327-452, 819-852) ----
+ * kval: K = -df/dx in the CSR layout of orc_csr_pattern (nnz values, via kid); aval: the system
+ * matrix (h + beta) h K + (1 + alpha h) M on the same pattern; b (3N): h f + M v with
+ * f = elastic (SVD route) + gravity, fixed nodes 1e8 (rest - x).  rest, gforce: 3N AoS;
+ * x, v, ext: 3N AoS (ext may be NULL). */
+void orc_implicit_system_f32(int nn, const float *rest, int nt, const int *tets, const int *mask,
+                             const float *dminv, const float *an, const float *mass, const float *gforce,
+                             float mu, float lam, int material, const float *x, const float *v, const float *ext,
+                             float h, float alpha, float beta, long nnz, const int *kid, const int *diag_idx,
+                             double *kval, double *aval, double *b);
+void orc_implicit_system_f64(int nn, const float *rest, int nt, const int *tets, const int *mask,
+                             const float *dminv, const float *an, const float *mass, const float *gforce,
+                             float mu, float lam, int material, const double *x, const double *v,
+                             const double *ext, float h, float alpha, float beta, long nnz, const int *kid,
+                             const int *diag_idx, double *kval, double *aval, double *b);
+
 /* ---- adjacency (IsotropicMaterial.cpp:554-644, IsotropicMaterial.h:120-138; ProjDynamic.cpp:25-45) ---- */
 /* Pattern of the 3N x 3N stiffness matrix: returns nnz; rowptr (3N+1) and colidx (nnz) are
  * written when non-NULL (call once with NULLs to size).  diag_idx (3N) and kid (48T) as the

@@ -77,6 +77,10 @@ def lib():
         L.orc_csr_pattern.argtypes = [C.c_int, C.c_int, P, P, P, P, P]
         L.orc_csr_pattern.restype = C.c_long
         L.orc_incidence.argtypes = [C.c_int, C.c_int, P, P, P]
+        for sfx in ("f32", "f64"):
+            getattr(L, "orc_implicit_system_" + sfx).argtypes = [
+                C.c_int, P, C.c_int, P, P, P, P, P, P, C.c_float, C.c_float, C.c_int, P, P, P,
+                C.c_float, C.c_float, C.c_float, C.c_long, P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -185,6 +189,30 @@ def incidence(nn, tets):
     ent = np.zeros(4 * len(tets), np.int32)
     lib().orc_incidence(nn, len(tets), _p(tets), _p(off), _p(ent))
     return off, ent
+
+
+def implicit_system(rest, tets, mask, pre, mu, lam, x, v, h=0.03, alpha=0.02, beta=0.02, ext=None,
+                    material=MAT_NEOHOOKEAN, precision=64):
+    """Backward-Euler system of the reference's GPU implicit path: (rowptr, col, K values,
+    system-matrix values, right-hand side), CSR in the reference's layout."""
+    dt = np.float32 if precision == 32 else np.float64
+    rest = np.ascontiguousarray(rest, np.float32).reshape(-1, 3)
+    tets = np.ascontiguousarray(tets, np.int32).reshape(-1, 4)
+    mask = np.ascontiguousarray(mask, np.int32)
+    nn, nt = len(rest), len(tets)
+    rowptr, col, diag, kid = csr_pattern(nn, tets)
+    nnz = len(col)
+    x = np.ascontiguousarray(x, dt).reshape(-1)
+    v = np.ascontiguousarray(v, dt).reshape(-1)
+    e = None if ext is None else np.ascontiguousarray(ext, dt).reshape(-1)
+    kval = np.zeros(nnz)
+    aval = np.zeros(nnz)
+    b = np.zeros(3 * nn)
+    fn = lib().orc_implicit_system_f32 if precision == 32 else lib().orc_implicit_system_f64
+    fn(nn, _p(rest), nt, _p(tets), _p(mask), _p(pre["dminv"]), _p(pre["an"]), _p(pre["mass"]),
+       _p(np.ascontiguousarray(pre["gforce"], np.float32)), mu, lam, material, _p(x), _p(v), _p(e), h, alpha, beta,
+       nnz, _p(kid), _p(diag), _p(kval), _p(aval), _p(b))
+    return rowptr, col, kval, aval, b
 
 
 class Solver:

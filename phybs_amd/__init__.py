@@ -49,6 +49,31 @@ class StepStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ImplicitParams(C.Structure):
+    _fields_ = [
+        ("damping_alpha", C.c_float), ("damping_beta", C.c_float), ("cg_tolerance", C.c_double),
+        ("cg_max_iterations", C.c_int), ("warm_start", C.c_int),
+    ]
+
+
+class ImplicitStats(C.Structure):
+    _fields_ = [
+        ("cg_iterations", C.c_int), ("converged", C.c_int), ("residual0", C.c_double), ("residual", C.c_double),
+        ("rhs_norm", C.c_double), ("gpu_ms", C.c_float),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def implicit_params(**kw):
+    p = ImplicitParams()
+    _check(lib().pbs_implicit_default_params(C.byref(p)))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
 # name -> (restype, argtypes); every symbol include/phybs.h declares.
 _P = C.c_void_p
 SIGNATURES = {
@@ -87,6 +112,9 @@ SIGNATURES = {
     "pbs_node_read": (C.c_int, [C.c_char_p, C.c_int, _P]),
     "pbs_csr_pattern": (C.c_int64, [C.c_int, C.c_int, _P, _P, _P, _P, _P]),
     "pbs_incidence": (C.c_int, [C.c_int, C.c_int, _P, _P, _P]),
+    "pbs_implicit_default_params": (C.c_int, [_P]),
+    "pbs_implicit_step": (C.c_int, [_P, _P, _P, _P, _P]),
+    "pbs_stiffness_matrix": (C.c_int, [_P, C.c_int64, _P]),
     "pbs_face_ring": (C.c_int64, [C.c_int, C.c_int, _P, _P, _P]),
     "pbs_pick_face": (C.c_int, [C.c_int, _P, C.c_int, _P, _P, _P, _P]),
     "pbs_drag_face_ring": (C.c_int, [C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_int, _P, _P]),
@@ -350,6 +378,22 @@ class Solver:
         out = np.zeros((self.nn, 3), np.float32) if want_positions else None
         _check(lib().pbs_step(self.h, _p(e), _p(out), C.byref(st)))
         return (st, out) if want_positions else st
+
+    def implicit_step(self, ext=None, params=None, want_positions=False):
+        """One backward-Euler step; returns ImplicitStats (and the new positions)."""
+        st = ImplicitStats()
+        p = params if params is not None else implicit_params()
+        e = None if ext is None else _f32(ext)
+        out = np.zeros((self.nn, 3), np.float32) if want_positions else None
+        _check(lib().pbs_implicit_step(self.h, C.byref(p), _p(e), _p(out), C.byref(st)))
+        return (st, out) if want_positions else st
+
+    def stiffness_matrix(self, tets):
+        """K = -df/dx at the current positions: (rowptr, col, values) in the reference's CSR layout."""
+        rowptr, col, _, _ = csr_pattern(self.nn, tets)
+        val = np.zeros(len(col))
+        _check(lib().pbs_stiffness_matrix(self.h, len(col), _p(val)))
+        return rowptr, col, val
 
     def do_descent_opt(self, h, iterations, ext):
         e = _f32(ext)

@@ -311,6 +311,8 @@ static void free_all(pbs_solver *s) {
   graph_release(s);
   learn_release(s->learn);
   s->learn = nullptr;
+  implicit_release(s->implicit);
+  s->implicit = nullptr;
   for (auto e : s->gev) (void)hipEventDestroy(e);
   s->gev.clear();
   if (s->ev0) (void)hipEventDestroy(s->ev0);

@@ -1,0 +1,334 @@
+// pbs_implicit.cpp -- C ABI of the backward-Euler step (include/phybs.h, "backward Euler"):
+// matrix layout construction, the conjugate-gradient driver (HIP graphs of CG iterations
+// replayed until the device-side test stops them) and the stiffness-matrix export.
+#include "pbs_implicit.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pbs_solver.h"
+
+namespace pbs {
+
+void implicit_release(ImplicitState *S) {
+  if (!S) return;
+  if (S->graph) (void)hipGraphExecDestroy(S->graph);
+  if (S->tgraph) (void)hipGraphExecDestroy(S->tgraph);
+  for (auto e : S->ev) (void)hipEventDestroy(e);
+  for (void *q : S->allocs) (void)hipFree(q);
+  delete S;
+}
+
+}  // namespace pbs
+
+namespace {
+
+constexpr int kGraphIters = 16;  // CG iterations per graph replay
+
+template <typename T> int ialloc(ImplicitState *S, T **p, size_t n) {
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return set_error(PBS_ERR_HIP, std::string("hipMalloc(") + std::to_string(n * sizeof(T)) + " B): " + hipGetErrorString(e));
+  }
+  S->allocs.push_back(q);
+  *p = (T *)q;
+  return PBS_OK;
+}
+
+// Block rows of K in solver order: a node's columns are the nodes sharing a tet with it,
+// ascending, itself included (the block pattern of allocateGlobalStiffnessMatrix,
+// IsotropicMaterial.cpp:554-644), laid out as SELL-64.
+int implicit_get(pbs_solver *s, ImplicitState **out) {
+  if (s->implicit) {
+    *out = s->implicit;
+    return PBS_OK;
+  }
+  if (s->ninst != 1) return set_error(PBS_ERR_STATE, "backward-Euler step: not available on a batched solver");
+  const int N = s->nn;
+  const TilePlan &P = s->plan;
+  std::vector<int> row_off(N + 1, 0), cols, tmp;
+  cols.reserve((size_t)N * 16);
+  for (int n = 0; n < N; ++n) {
+    tmp.clear();
+    for (int e = P.inc_off[n]; e < P.inc_off[n + 1]; ++e) {
+      const int t = P.inc[e] >> 2;
+      for (int c = 0; c < 4; ++c) tmp.push_back(P.tets_new[4 * (size_t)t + c]);
+    }
+    tmp.push_back(n);
+    std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    cols.insert(cols.end(), tmp.begin(), tmp.end());
+    row_off[n + 1] = (int)cols.size();
+  }
+  ImplicitState *S = new ImplicitState();
+  S->nslice = (N + 63) / 64;
+  S->slice_off_h.assign(S->nslice + 1, 0);
+  S->row_len_h.resize(N);
+  for (int n = 0; n < N; ++n) S->row_len_h[n] = row_off[n + 1] - row_off[n];
+  for (int sl = 0; sl < S->nslice; ++sl) {
+    int w = 0;
+    for (int n = 64 * sl; n < std::min(N, 64 * sl + 64); ++n) w = std::max(w, S->row_len_h[n]);
+    S->slice_off_h[sl + 1] = S->slice_off_h[sl] + w;
+  }
+  S->slots = 64L * S->slice_off_h[S->nslice];
+  S->bcol_h.assign(S->slots, 0);
+  std::vector<int> diag(N);
+  for (int n = 0; n < N; ++n) {
+    const long g0 = S->slice_off_h[n >> 6], w = S->slice_off_h[(n >> 6) + 1] - g0;
+    const int lane = n & 63;
+    for (long j = 0; j < w; ++j) {
+      const bool real = j < S->row_len_h[n];
+      const int c = real ? cols[row_off[n] + j] : n;
+      S->bcol_h[(g0 + j) * 64 + lane] = c;
+      if (real && c == n) diag[n] = (int)((g0 + j) * 64 + lane);
+    }
+  }
+  const size_t rsz = s->p.precision == PBS_FP64 ? 8 : 4;
+  const size_t n3 = 3 * (size_t)N;
+  const int nb = (N + 255) / 256;
+  IeDev &d = S->dev;
+  d.n = N;
+  d.nslice = S->nslice;
+  int *so = nullptr, *rl = nullptr, *bc = nullptr, *dsl = nullptr;
+  char *A = nullptr;
+  int rc = ialloc(S, &so, S->nslice + 1);
+  if (!rc) rc = ialloc(S, &rl, N);
+  if (!rc) rc = ialloc(S, &bc, S->slots);
+  if (!rc) rc = ialloc(S, &dsl, N);
+  if (!rc) rc = ialloc(S, &A, 9 * rsz * S->slots);
+  double **vec[8] = {&d.b, &d.x, &d.r, &d.z, &d.p0, &d.p1, &d.Ap, &d.dinv};
+  for (int i = 0; i < 8 && !rc; ++i) rc = ialloc(S, vec[i], n3);
+  if (!rc) rc = ialloc(S, &d.part, 4 * (size_t)nb);
+  if (!rc) rc = ialloc(S, &d.ctrl, 1);
+  if (rc) {
+    implicit_release(S);
+    return rc;
+  }
+  d.slice_off = so;
+  d.row_len = rl;
+  d.bcol = bc;
+  d.diag_slot = dsl;
+  d.A = A;
+  hipError_t e = hipMemcpy(so, S->slice_off_h.data(), sizeof(int) * (S->nslice + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(rl, S->row_len_h.data(), sizeof(int) * N, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(bc, S->bcol_h.data(), sizeof(int) * S->slots, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dsl, diag.data(), sizeof(int) * N, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    implicit_release(S);
+    return set_error(PBS_ERR_HIP, std::string("backward-Euler setup: ") + hipGetErrorString(e));
+  }
+  s->implicit = S;
+  *out = S;
+  return PBS_OK;
+}
+
+// One graph holds kGraphIters CG iterations; the timed variant brackets each SpMV and each
+// vector update with external event records.
+template <typename R> int cg_graph(pbs_solver *s, ImplicitState *S, int timed) {
+  hipGraphExec_t &g = timed ? S->tgraph : S->graph;
+  const double key[2] = {S->dev.tol2, (double)S->dev.maxit};
+  if (g && std::memcmp(key, S->graph_key, sizeof(key)) == 0) return PBS_OK;
+  if (S->graph) (void)hipGraphExecDestroy(S->graph);
+  if (S->tgraph) (void)hipGraphExecDestroy(S->tgraph);
+  S->graph = S->tgraph = nullptr;
+  std::memcpy(S->graph_key, key, sizeof(key));
+  for (int t = 0; t < 2; ++t) {
+    if (t == 1 && !timed) break;
+    while (t == 1 && S->ev.size() < 4 * (size_t)kGraphIters + 2) {
+      hipEvent_t ev;
+      HIPCHK(hipEventCreate(&ev));
+      S->ev.push_back(ev);
+    }
+    HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+    hipError_t err = hipSuccess;
+    for (int i = 0; i < kGraphIters && err == hipSuccess; ++i) {
+      if (t) err = hipEventRecordWithFlags(S->ev[4 * i], s->st, hipEventRecordExternal);
+      IeLaunch<R>::spmv(S->dev, s->st);
+      if (t && err == hipSuccess) err = hipEventRecordWithFlags(S->ev[4 * i + 1], s->st, hipEventRecordExternal);
+      IeLaunch<R>::alpha(S->dev, s->st);
+      if (t && err == hipSuccess) err = hipEventRecordWithFlags(S->ev[4 * i + 2], s->st, hipEventRecordExternal);
+      IeLaunch<R>::update(S->dev, s->st);
+      if (t && err == hipSuccess) err = hipEventRecordWithFlags(S->ev[4 * i + 3], s->st, hipEventRecordExternal);
+      IeLaunch<R>::beta(S->dev, s->st);
+      if (err == hipSuccess) err = hipGetLastError();
+    }
+    hipGraph_t gr = nullptr;
+    hipError_t e2 = hipStreamEndCapture(s->st, &gr);
+    if (err == hipSuccess) err = e2;
+    hipGraphExec_t ex = nullptr;
+    if (err == hipSuccess) err = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    if (gr) (void)hipGraphDestroy(gr);
+    if (err != hipSuccess) return set_error(PBS_ERR_HIP, std::string("CG graph capture: ") + hipGetErrorString(err));
+    (t ? S->tgraph : S->graph) = ex;
+  }
+  return PBS_OK;
+}
+
+int check_iparams(const pbs_implicit_params *ip) {
+  if (!ip) return set_error(PBS_ERR_ARG, "null implicit parameters");
+  if (!(ip->cg_tolerance > 0) || !(ip->cg_tolerance < 1)) return set_error(PBS_ERR_ARG, "cg_tolerance must lie in (0, 1)");
+  if (ip->cg_max_iterations < 1) return set_error(PBS_ERR_ARG, "cg_max_iterations must be >= 1");
+  if (!(ip->damping_alpha >= 0) || !(ip->damping_beta >= 0)) return set_error(PBS_ERR_ARG, "damping must be >= 0");
+  return PBS_OK;
+}
+
+template <typename R>
+int implicit_step_impl(pbs_solver *s, const pbs_implicit_params *ip, const float *ext, float *out,
+                       pbs_implicit_stats *stats) {
+  ImplicitState *S = nullptr;
+  int rc = implicit_get(s, &S);
+  if (rc) return rc;
+  DevMesh<R> &m = mesh<R>(s);
+  IeDev &d = S->dev;
+  d.tol2 = ip->cg_tolerance * ip->cg_tolerance;
+  d.maxit = ip->cg_max_iterations;
+  const bool timed = s->prof_period > 0;
+  rc = cg_graph<R>(s, S, timed ? 1 : 0);
+  if (rc) return rc;
+  if (ext) {
+    HIPCHK(hipMemcpyAsync(s->stage, ext, sizeof(float) * 3 * (size_t)s->nn, hipMemcpyHostToDevice, s->st));
+    aos_to_planes_f(s->stage, m.ext, m.perm, s->nn, s->st);
+  }
+  IeAsm a{};
+  a.h = s->p.dt;
+  a.alpha = ip->damping_alpha;
+  a.beta = ip->damping_beta;
+  a.raw = 0;
+  a.warm = ip->warm_start ? 1 : 0;
+  a.has_ext = ext ? 1 : 0;
+  HIPCHK(hipEventRecord(s->ev0, s->st));
+  hipEvent_t *ea = timed ? &S->ev[4 * kGraphIters] : nullptr;  // events around the assembly
+  if (ea) HIPCHK(hipEventRecord(ea[0], s->st));
+  IeLaunch<R>::assemble(m, s->td, s->dp, d, a, s->st);
+  HIPCHK(hipGetLastError());
+  if (ea) HIPCHK(hipEventRecord(ea[1], s->st));
+  IeLaunch<R>::start(d, s->st);
+  HIPCHK(hipGetLastError());
+  IeCtrl c{};
+  HIPCHK(hipMemcpyAsync(&c, d.ctrl, sizeof(IeCtrl), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  if (ea) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ea[0], ea[1]) == hipSuccess) {
+      s->kt_ms[K_IE_ASSEMBLE] += ms;
+      s->kt_n[K_IE_ASSEMBLE] += 1;
+    }
+  }
+  long replays = 0;
+  const long max_replays = (ip->cg_max_iterations + kGraphIters - 1) / kGraphIters + 1;
+  while (!c.done) {
+    if (replays++ >= max_replays) return set_error(PBS_ERR_STATE, "CG loop did not stop within its iteration cap");
+    const int k_before = c.k;
+    HIPCHK(hipGraphLaunch(timed ? S->tgraph : S->graph, s->st));
+    HIPCHK(hipMemcpyAsync(&c, d.ctrl, sizeof(IeCtrl), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    if (timed) {  // iterations that ran: k advanced by one per completed iteration (+1 if stopped in alpha)
+      const int ran = std::min(kGraphIters, c.k - k_before + (c.indefinite ? 1 : 0));
+      for (int i = 0; i < ran; ++i) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, S->ev[4 * i], S->ev[4 * i + 1]) == hipSuccess) {
+          s->kt_ms[K_IE_SPMV] += ms;
+          s->kt_n[K_IE_SPMV] += 1;
+        }
+        if (c.indefinite && i == ran - 1) break;
+        if (hipEventElapsedTime(&ms, S->ev[4 * i + 2], S->ev[4 * i + 3]) == hipSuccess) {
+          s->kt_ms[K_IE_UPDATE] += ms;
+          s->kt_n[K_IE_UPDATE] += 1;
+        }
+      }
+    }
+  }
+  if (c.indefinite)
+    return set_error(PBS_ERR_STATE, "backward-Euler step: the system matrix is not positive definite at this state");
+  IeLaunch<R>::finish(m, s->dp, s->p.dt, d, s->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(s->ev1, s->st));
+  if (out) {
+    Launch<R>::planes_to_aos(m.xpos, s->stage, m.perm, s->nn, s->st);
+    HIPCHK(hipMemcpyAsync(out, s->stage, sizeof(float) * 3 * (size_t)s->nn, hipMemcpyDeviceToHost, s->st));
+  }
+  HIPCHK(hipStreamSynchronize(s->st));
+  if (stats) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
+    std::memset(stats, 0, sizeof(*stats));
+    stats->cg_iterations = c.k;
+    stats->converged = c.converged;
+    stats->residual0 = std::sqrt(c.rz0);
+    stats->residual = std::sqrt(c.rz);
+    stats->rhs_norm = std::sqrt(c.bb);
+    stats->gpu_ms = ms;
+  }
+  return PBS_OK;
+}
+
+template <typename R> int stiffness_impl(pbs_solver *s, int64_t nnz, double *csr_val) {
+  ImplicitState *S = nullptr;
+  int rc = implicit_get(s, &S);
+  if (rc) return rc;
+  const int N = s->nn;
+  int64_t total = 0;
+  for (int n = 0; n < N; ++n) total += 9L * S->row_len_h[n];
+  if (nnz != total) return set_error(PBS_ERR_ARG, "nnz does not match the stiffness pattern (" + std::to_string(total) + ")");
+  IeAsm a{};
+  a.raw = 1;
+  IeLaunch<R>::assemble(mesh<R>(s), s->td, s->dp, S->dev, a, s->st);
+  HIPCHK(hipGetLastError());
+  std::vector<R> A(9 * (size_t)S->slots);
+  HIPCHK(hipMemcpyAsync(A.data(), S->dev.A, sizeof(R) * A.size(), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  // reference CSR order: row 3a+d of caller node a, columns 3b+e with b ascending in caller ids
+  const std::vector<int> &perm = s->plan.perm, &iperm = s->plan.iperm;
+  std::vector<int64_t> row_start(N + 1, 0);
+  for (int a2 = 0; a2 < N; ++a2) row_start[a2 + 1] = row_start[a2] + 9L * S->row_len_h[iperm[a2]];
+  std::vector<std::pair<int, long>> nb;
+  for (int a2 = 0; a2 < N; ++a2) {
+    const int n = iperm[a2];
+    const long g0 = S->slice_off_h[n >> 6];
+    const int lane = n & 63, len = S->row_len_h[n];
+    nb.clear();
+    for (int j = 0; j < len; ++j) nb.push_back({perm[S->bcol_h[(g0 + j) * 64 + lane]], g0 + j});
+    std::sort(nb.begin(), nb.end());
+    const int64_t rs = row_start[a2];
+    const int64_t rl = 3L * len;  // values per CSR row
+    for (int j = 0; j < len; ++j)
+      for (int dd = 0; dd < 3; ++dd)
+        for (int ee = 0; ee < 3; ++ee)
+          csr_val[rs + dd * rl + 3L * j + ee] = (double)A[((size_t)nb[j].second * 9 + 3 * dd + ee) * 64 + lane];
+  }
+  return PBS_OK;
+}
+
+}  // namespace
+
+extern "C" int pbs_implicit_default_params(pbs_implicit_params *p) {
+  if (!p) return set_error(PBS_ERR_ARG, "null argument");
+  std::memset(p, 0, sizeof(*p));
+  p->damping_alpha = 0.02f;  // m_dumpingAlpha, m_dumpingBelta (CUDAInterface.cu:644)
+  p->damping_beta = 0.02f;
+  p->cg_tolerance = 1e-6;
+  p->cg_max_iterations = 10000;  // conjugateGradient's cap (CUDALinearSolvers.cu)
+  p->warm_start = 1;
+  return PBS_OK;
+}
+
+extern "C" int pbs_implicit_step(pbs_solver *s, const pbs_implicit_params *ip, const float *ext, float *out,
+                                 pbs_implicit_stats *stats) {
+  if (!s) return set_error(PBS_ERR_ARG, "null handle");
+  int rc = check_iparams(ip);
+  if (rc) return rc;
+  return s->p.precision == PBS_FP64 ? implicit_step_impl<double>(s, ip, ext, out, stats)
+                                    : implicit_step_impl<float>(s, ip, ext, out, stats);
+}
+
+extern "C" int pbs_stiffness_matrix(pbs_solver *s, int64_t nnz, double *csr_val) {
+  if (!s || !csr_val) return set_error(PBS_ERR_ARG, "null argument");
+  return s->p.precision == PBS_FP64 ? stiffness_impl<double>(s, nnz, csr_val) : stiffness_impl<float>(s, nnz, csr_val);
+}

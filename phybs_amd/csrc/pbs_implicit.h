@@ -1,0 +1,71 @@
+// pbs_implicit.h -- backward-Euler step (the reference's GPU implicit path): per-node assembly
+// of the global stiffness matrix and right-hand side, and a Jacobi-preconditioned conjugate-
+// gradient solve, all on the device.  Kernels in pbs_implicit.hip, host side in pbs_implicit.cpp.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "pbs_internal.h"
+
+namespace pbs {
+
+// Matrix layout: 3x3 blocks, one block row per node (solver order), stored as SELL-64: rows are
+// grouped in slices of 64 (one wavefront), a slice is as wide as its longest row, and slot j of
+// the slice's row l sits at (slice_off[s] + j) * 64 + l; block values are spread over 9
+// consecutive slot groups so that each of the 9 loads of a wavefront is one contiguous 256-byte
+// (FP32) access.  Padding slots have column = the row itself and zero values.
+struct IeCtrl {
+  double rz, pAp, alpha, beta, rr, bb, rr0, rz0;  // bb = b.D^-1.b
+  int k, done, converged, indefinite;
+};
+
+struct IeDev {
+  int n, nslice, maxit, pad;
+  double tol2;                     // (relative residual tolerance)^2
+  const int *slice_off;            // nslice + 1, in slot groups of 64
+  const int *row_len;              // N
+  const int *bcol;                 // slots
+  const int *diag_slot;            // N: slot index of each row's diagonal block
+  void *A;                         // 9 x slots values of the solver's real type
+  double *b, *x, *r, *z, *p0, *p1, *Ap, *dinv;  // 3N, node-major (x, y, z of a node together)
+  double *part;                    // 4 partial sums per 256-row block
+  IeCtrl *ctrl;
+};
+
+// Constants of one assembly (scales of the system matrix, fixed-node spring, damping).
+struct IeAsm {
+  float h, alpha, beta;  // time step, mass and stiffness damping coefficients
+  int raw;               // 1: store K itself (no scaling, no mass), skip the right-hand side
+  int warm;              // 1: start the solve from the current velocities, 0: from zero
+  int has_ext;
+};
+
+template <typename R> struct IeLaunch {
+  static void assemble(const DevMesh<R> &m, const TileDev &td, const DevParams &p, const IeDev &d, const IeAsm &a,
+                       hipStream_t st);
+  static void start(const IeDev &d, hipStream_t st);
+  // one CG iteration = spmv, alpha, update, beta (each a kernel)
+  static void spmv(const IeDev &d, hipStream_t st);
+  static void alpha(const IeDev &d, hipStream_t st);
+  static void update(const IeDev &d, hipStream_t st);
+  static void beta(const IeDev &d, hipStream_t st);
+  static void finish(const DevMesh<R> &m, const DevParams &p, float h, const IeDev &d, hipStream_t st);
+};
+
+struct ImplicitState {
+  int nslice = 0;
+  long slots = 0;
+  IeDev dev{};
+  std::vector<int> slice_off_h, row_len_h, bcol_h;  // host copies (stiffness export)
+  hipGraphExec_t graph = nullptr;
+  int graph_iters = 0;
+  double graph_key[2] = {0, 0};                    // tol2, maxit the graph was built with
+  hipGraphExec_t tgraph = nullptr;                 // timed variant: events around each SpMV
+  std::vector<hipEvent_t> ev;
+  std::vector<void *> allocs;
+};
+
+void implicit_release(ImplicitState *S);
+
+}  // namespace pbs

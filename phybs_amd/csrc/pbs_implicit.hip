@@ -1,0 +1,463 @@
+// pbs_implicit.hip -- kernels of the backward-Euler step (computeInnerforces,
+// computeGlobalStiffnessMatrix and doBackEuler, CUDAInterface.cu:784-884, as driven by
+// fvm.cpp:179-183).
+//
+// The reference scatters 144 float atomics per tet into a CSR matrix (computeGlobalK,
+// CUDAInterface.cu:234-271) and solves with a sparse Cholesky factorisation.  Here one thread
+// owns one block row (a node): it walks the node's incident tets in a fixed order, evaluates the
+// 3x12 stiffness strip of its corner in closed form and accumulates it into its own SELL-64
+// slots, so the assembly is deterministic and atomic-free; the solve is a Jacobi-preconditioned
+// conjugate gradient in FP64 whose scalars stay on the device.
+#include "pbs_device.h"
+#include "pbs_implicit.h"
+
+namespace pbs {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ size_t val_idx(long group, int q, int lane) {
+  return ((size_t)group * 9 + q) * 64 + lane;
+}
+
+// Sum of v over the 256 threads of the block in a fixed order (wave tree, then the 4 waves).
+__device__ __forceinline__ double block_sum256(double v, double *lds) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) lds[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) t = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+  __syncthreads();
+  return t;
+}
+
+// Sum of component c of the per-block partials, by one block of 1024 threads in a fixed order.
+__device__ __forceinline__ double partial_sum(const double *part, int nb, int c, double *lds) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 1024) v += part[4 * (size_t)i + c];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) lds[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < 16; ++i) t += lds[i];
+  __syncthreads();
+  return t;
+}
+
+// Slot (within the row) of column `col`: the row's columns are ascending.
+__device__ __forceinline__ int find_col(const int *__restrict__ bcol, int g0, int lane, int len, int col) {
+  int lo = 0, hi = len - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (bcol[(size_t)(g0 + mid) * 64 + lane] < col) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Assembly.  For node n at corner i of an incident tet, with shape vectors c_m (rows of Dm^-1,
+// c_0 = -(c_1 + c_2 + c_3)), g_m = F^-T c_m and V the rest volume, the stiffness block
+// K(i,j)[d][e] = -d f_i[d] / d x_j[e] is
+//   neo-Hookean: V [ mu (c_i . c_j) delta_de + (mu - lambda ln J) g_j[d] g_i[e] + lambda g_i[d] g_j[e] ]
+//   stretch:     V [ 2 mu ((c_i . c_j) delta_de - sum_{k<l} (a^d_kl . c_i)(a^e_kl . c_j) / (s_k + s_l))
+//                    + lambda ((2J - 1) J g_i[d] g_j[e] - (J - 1) J g_i[e] g_j[d]) ]
+// with a^d_kl[b] = U_dk V_bl - U_dl V_bk (F = U diag(s) V^T): the closed form of the
+// derivative the reference evaluates through the SVD and Xu et al. 2015's 2x2 solves
+// (compute_dfdx, CUDAInterface.cu:101-219).  The elastic force of the corner comes from the same
+// per-tet routine as the descent solver's.
+// ---------------------------------------------------------------------------------------------
+template <typename R, int MAT>
+__global__ void __launch_bounds__(kBlock)
+    k_ie_assemble(DevParams p, IeAsm a, IeDev d, const int4 *__restrict__ tets, const float *__restrict__ dmi,
+                  const float *__restrict__ vol, const int *__restrict__ inc_off, const int *__restrict__ inc,
+                  const R *__restrict__ xpos, const R *__restrict__ vel, const float *__restrict__ rest,
+                  const float *__restrict__ mass, const float *__restrict__ gforce, const float *__restrict__ ext,
+                  const int *__restrict__ mask, const float *__restrict__ inst_mu, const float *__restrict__ inst_lam) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= d.n) return;
+  R *__restrict__ A = (R *)d.A;
+  const int N = p.nn, T = p.nt;
+  const int lane = n & 63;
+  const int g0 = d.slice_off[n >> 6];
+  const int width = d.slice_off[(n >> 6) + 1] - g0;
+  const int len = d.row_len[n];
+  for (int j = 0; j < width; ++j)
+#pragma unroll
+    for (int q = 0; q < 9; ++q) A[val_idx(g0 + j, q, lane)] = (R)0;
+  const R mu = (R)inst_mu[0], lam = (R)inst_lam[0];
+  R f[3] = {(R)0, (R)0, (R)0};
+  for (int e = inc_off[n]; e < inc_off[n + 1]; ++e) {
+    const int t = inc[e] >> 2, ci = inc[e] & 3;
+    const int4 q4 = tets[t];
+    const int qn[4] = {q4.x, q4.y, q4.z, q4.w};
+    R Dmi[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Dmi[k] = (R)dmi[(size_t)k * T + t];
+    const R V = (R)vol[t];
+    R X[4][3];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) X[c][k] = xpos[(size_t)k * N + qn[c]];
+    R Ds[9], F[9];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      Ds[3 * k + 0] = X[1][k] - X[0][k];
+      Ds[3 * k + 1] = X[2][k] - X[0][k];
+      Ds[3 * k + 2] = X[3][k] - X[0][k];
+    }
+    mul3(Ds, Dmi, F);
+    R f12[12], en;
+    if (MAT == 0) neo_tet<R, false>(F, Dmi, V, mu, lam, f12, en, nullptr);
+    else stretch_tet<R, false>(F, Dmi, V, mu, lam, f12, en, nullptr);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) f[k] += f12[3 * ci + k];
+    R C[9];
+    const R J = adj3(F, C);
+    const R iJ = (R)1 / J;
+    R c[4][3], g[4][3];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      shape_vec(Dmi, m, c[m]);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) g[m][k] = (c[m][0] * C[k] + c[m][1] * C[3 + k] + c[m][2] * C[6 + k]) * iJ;
+    }
+    R ci3[3], gi[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {  // corner ci's vectors, selected without dynamic register indexing
+      ci3[k] = ci == 0 ? c[0][k] : ci == 1 ? c[1][k] : ci == 2 ? c[2][k] : c[3][k];
+      gi[k] = ci == 0 ? g[0][k] : ci == 1 ? g[1][k] : ci == 2 ? g[2][k] : g[3][k];
+    }
+    R k1, k2;          // coefficients of g_j[d] g_i[e] and g_i[d] g_j[e]
+    R ai[3][3] = {};   // stretch: (a^d_kl . c_i) for the pairs kl = 01, 02, 12
+    R aw[3] = {};      // stretch: 1 / (s_k + s_l)
+    R U[9] = {}, Vv[9] = {};
+    if (MAT == 0) {
+      const R lj = dlog(J);
+      k1 = mu - lam * lj;
+      k2 = lam;
+    } else {
+      R S[3];
+      svd3(F, U, S, Vv);
+      k1 = -lam * (J - (R)1) * J;
+      k2 = lam * ((R)2 * J - (R)1) * J;
+      const int kk[3] = {0, 0, 1}, ll[3] = {1, 2, 2};
+      const R lim = (sizeof(R) == 4 ? (R)1e-6 : (R)1e-12);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const R den = S[kk[q]] + S[ll[q]];
+        aw[q] = (R)1 / (den > lim ? den : lim);
+#pragma unroll
+        for (int dd = 0; dd < 3; ++dd) {
+          R acc = (R)0;
+#pragma unroll
+          for (int b = 0; b < 3; ++b)
+            acc += (U[3 * dd + kk[q]] * Vv[3 * b + ll[q]] - U[3 * dd + ll[q]] * Vv[3 * b + kk[q]]) * ci3[b];
+          ai[q][dd] = acc;
+        }
+      }
+    }
+    const R mu2 = (MAT == 0) ? mu : (R)2 * mu;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int slot_g = g0 + find_col(d.bcol, g0, lane, len, qn[m]);
+      const R cc = ci3[0] * c[m][0] + ci3[1] * c[m][1] + ci3[2] * c[m][2];
+      R aj[3][3];
+      if (MAT != 0) {
+        const int kk[3] = {0, 0, 1}, ll[3] = {1, 2, 2};
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+          for (int ee = 0; ee < 3; ++ee) {
+            R acc = (R)0;
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+              acc += (U[3 * ee + kk[q]] * Vv[3 * b + ll[q]] - U[3 * ee + ll[q]] * Vv[3 * b + kk[q]]) * c[m][b];
+            aj[q][ee] = acc;
+          }
+      }
+#pragma unroll
+      for (int dd = 0; dd < 3; ++dd)
+#pragma unroll
+        for (int ee = 0; ee < 3; ++ee) {
+          R kv = (dd == ee ? mu2 * cc : (R)0) + k1 * g[m][dd] * gi[ee] + k2 * gi[dd] * g[m][ee];
+          if (MAT != 0) {
+            R rot = (R)0;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) rot += ai[q][dd] * aj[q][ee] * aw[q];
+            kv -= mu2 * rot;
+          }
+          A[val_idx(slot_g, 3 * dd + ee, lane)] += V * kv;
+        }
+    }
+  }
+  if (a.raw) return;
+  // system matrix (h + beta) h K + (1 + alpha h) M (doBackEuler, CUDAInterface.cu:845-851)
+  const R s1 = (R)a.h + (R)a.beta, s2 = (R)a.h;
+  for (int j = 0; j < len; ++j)
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      const size_t ix = val_idx(g0 + j, q, lane);
+      A[ix] = A[ix] * s1 * s2;
+    }
+  const R ms = (R)mass[n] * ((R)1 + (R)a.alpha * (R)a.h);
+  const int ds = d.diag_slot[n];
+  const long dg = ds >> 6;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const size_t ix = val_idx(dg, 4 * k, lane);
+    A[ix] += ms;
+    d.dinv[3 * (size_t)n + k] = 1.0 / (double)A[ix];
+  }
+  // right-hand side h f + m v (setRHSofLinearSystem :418-442) with the forces of
+  // addFixContraintsAndGravity (:327-343): fixed nodes a 1e8 spring to the rest position
+  const R h = (R)a.h;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const size_t i = (size_t)k * N + n;
+    R fk;
+    if (mask[n]) {
+      fk = (R)(1e8 * (double)((R)rest[i] - xpos[i]));
+    } else {
+      fk = f[k] + (R)gforce[i];
+      if (a.has_ext) fk += (R)ext[i];
+    }
+    const R vk = vel[i];
+    d.b[3 * (size_t)n + k] = (double)(fk * h + (R)mass[n] * vk);
+    d.x[3 * (size_t)n + k] = a.warm ? (double)vk : 0.0;
+  }
+}
+
+// r = b - A x0, z = D^-1 r, p = z; partials r.z, r.r, b.D^-1.b.  The stopping test uses the
+// Jacobi-scaled norms |D^-1/2 r| = sqrt(r.z) and |D^-1/2 b|: fixed nodes carry a 1e8 mass and
+// spring, and in the plain 2-norm their rows would swamp the free nodes' residual.
+template <typename R> __global__ void __launch_bounds__(kBlock) k_ie_start(IeDev d) {
+  __shared__ double lds[4];
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  double rz = 0.0, rr = 0.0, bb = 0.0;
+  if (n < d.n) {
+    const R *__restrict__ A = (const R *)d.A;
+    const int lane = n & 63, g0 = d.slice_off[n >> 6], len = d.row_len[n];
+    double y[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < len; ++j) {
+      const int col = d.bcol[(size_t)(g0 + j) * 64 + lane];
+      const double x0 = d.x[3 * (size_t)col], x1 = d.x[3 * (size_t)col + 1], x2 = d.x[3 * (size_t)col + 2];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        y[k] += (double)A[val_idx(g0 + j, 3 * k, lane)] * x0 + (double)A[val_idx(g0 + j, 3 * k + 1, lane)] * x1 +
+                (double)A[val_idx(g0 + j, 3 * k + 2, lane)] * x2;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const size_t i = 3 * (size_t)n + k;
+      const double b = d.b[i];
+      const double r = b - y[k];
+      const double z = d.dinv[i] * r;
+      d.r[i] = r;
+      d.z[i] = z;
+      d.p0[i] = z;
+      rz += r * z;
+      rr += r * r;
+      bb += b * b * d.dinv[i];
+    }
+  }
+  rz = block_sum256(rz, lds);
+  rr = block_sum256(rr, lds);
+  bb = block_sum256(bb, lds);
+  if (threadIdx.x == 0) {
+    d.part[4 * (size_t)blockIdx.x + 0] = rz;
+    d.part[4 * (size_t)blockIdx.x + 1] = rr;
+    d.part[4 * (size_t)blockIdx.x + 2] = bb;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_ie_start_ctrl(IeDev d, int nb) {
+  __shared__ double lds[16];
+  const double rz = partial_sum(d.part, nb, 0, lds);
+  const double rr = partial_sum(d.part, nb, 1, lds);
+  const double bb = partial_sum(d.part, nb, 2, lds);
+  if (threadIdx.x == 0) {
+    IeCtrl c = {};
+    c.rz = rz;
+    c.rr = rr;
+    c.rr0 = rr;
+    c.rz0 = rz;
+    c.bb = bb;
+    c.converged = (rz <= d.tol2 * bb) ? 1 : 0;
+    c.done = c.converged || d.maxit <= 0;
+    *d.ctrl = c;
+  }
+}
+
+// Iteration k: p_k = z + beta p_{k-1} (formed on the fly for every column, stored for the own
+// row), Ap = A p_k, partial p_k . Ap.
+template <typename R> __global__ void __launch_bounds__(kBlock) k_ie_spmv(IeDev d) {
+  __shared__ double lds[4];
+  const IeCtrl *c = d.ctrl;
+  if (c->done) return;
+  const int k = c->k;
+  const double beta = c->beta;
+  const double *__restrict__ pin = (k & 1) ? d.p0 : d.p1;  // p_{k-1} (k >= 1)
+  double *__restrict__ pout = (k & 1) ? d.p1 : d.p0;       // p_k
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  double pap = 0.0;
+  if (n < d.n) {
+    const R *__restrict__ A = (const R *)d.A;
+    const int lane = n & 63, g0 = d.slice_off[n >> 6], len = d.row_len[n];
+    double y[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < len; ++j) {
+      const int col = d.bcol[(size_t)(g0 + j) * 64 + lane];
+      double x0, x1, x2;
+      if (k == 0) {
+        x0 = pout[3 * (size_t)col];
+        x1 = pout[3 * (size_t)col + 1];
+        x2 = pout[3 * (size_t)col + 2];
+      } else {
+        x0 = d.z[3 * (size_t)col] + beta * pin[3 * (size_t)col];
+        x1 = d.z[3 * (size_t)col + 1] + beta * pin[3 * (size_t)col + 1];
+        x2 = d.z[3 * (size_t)col + 2] + beta * pin[3 * (size_t)col + 2];
+      }
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        y[q] += (double)A[val_idx(g0 + j, 3 * q, lane)] * x0 + (double)A[val_idx(g0 + j, 3 * q + 1, lane)] * x1 +
+                (double)A[val_idx(g0 + j, 3 * q + 2, lane)] * x2;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const size_t i = 3 * (size_t)n + q;
+      const double pk = k == 0 ? pout[i] : d.z[i] + beta * pin[i];
+      if (k != 0) pout[i] = pk;
+      d.Ap[i] = y[q];
+      pap += pk * y[q];
+    }
+  }
+  pap = block_sum256(pap, lds);
+  if (threadIdx.x == 0) d.part[4 * (size_t)blockIdx.x + 3] = pap;
+}
+
+__global__ void __launch_bounds__(1024) k_ie_alpha(IeDev d, int nb) {
+  __shared__ double lds[16];
+  if (d.ctrl->done) return;
+  const double pap = partial_sum(d.part, nb, 3, lds);
+  if (threadIdx.x == 0) {
+    IeCtrl *c = d.ctrl;
+    if (!(pap > 0.0)) {  // not positive definite (or a non-finite value): stop without a step
+      c->indefinite = 1;
+      c->done = 1;
+      c->alpha = 0.0;
+    } else {
+      c->pAp = pap;
+      c->alpha = c->rz / pap;
+    }
+  }
+}
+
+// x += alpha p_k, r -= alpha Ap, z = D^-1 r; partials r.z, r.r
+__global__ void __launch_bounds__(kBlock) k_ie_update(IeDev d) {
+  __shared__ double lds[4];
+  const IeCtrl *c = d.ctrl;
+  if (c->done) return;
+  const double alpha = c->alpha;
+  const double *__restrict__ pk = (c->k & 1) ? d.p1 : d.p0;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  double rz = 0.0, rr = 0.0;
+  if (n < d.n) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const size_t i = 3 * (size_t)n + q;
+      d.x[i] += alpha * pk[i];
+      const double r = d.r[i] - alpha * d.Ap[i];
+      const double z = d.dinv[i] * r;
+      d.r[i] = r;
+      d.z[i] = z;
+      rz += r * z;
+      rr += r * r;
+    }
+  }
+  rz = block_sum256(rz, lds);
+  rr = block_sum256(rr, lds);
+  if (threadIdx.x == 0) {
+    d.part[4 * (size_t)blockIdx.x + 0] = rz;
+    d.part[4 * (size_t)blockIdx.x + 1] = rr;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_ie_beta(IeDev d, int nb) {
+  __shared__ double lds[16];
+  if (d.ctrl->done) return;
+  const double rz = partial_sum(d.part, nb, 0, lds);
+  const double rr = partial_sum(d.part, nb, 1, lds);
+  if (threadIdx.x == 0) {
+    IeCtrl *c = d.ctrl;
+    c->k += 1;
+    c->rr = rr;
+    c->beta = rz / c->rz;
+    c->rz = rz;
+    c->converged = (rz <= d.tol2 * c->bb) ? 1 : 0;
+    c->done = c->converged || c->k >= d.maxit;
+  }
+}
+
+// v = solution, x += h v for every node (cublasSaxpy in doBackEuler, CUDAInterface.cu:873).
+template <typename R>
+__global__ void k_ie_finish(DevParams p, float h, IeDev d, R *__restrict__ xpos, R *__restrict__ vel) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= d.n) return;
+  const bool zero = d.ctrl->bb == 0.0;  // b = 0: the solution is 0 whatever the start
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const size_t i = (size_t)k * p.nn + n;
+    const R v = zero ? (R)0 : (R)d.x[3 * (size_t)n + k];
+    vel[i] = v;
+    xpos[i] = xpos[i] + (R)h * v;
+  }
+}
+
+inline int nb_of(int n) { return (n + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+template <typename R>
+void IeLaunch<R>::assemble(const DevMesh<R> &m, const TileDev &td, const DevParams &p, const IeDev &d,
+                           const IeAsm &a, hipStream_t st) {
+  if (p.material == 0)
+    hipLaunchKernelGGL((k_ie_assemble<R, 0>), dim3(nb_of(d.n)), dim3(kBlock), 0, st, p, a, d, m.tets, m.dmi, m.vol,
+                       m.inc_off, m.inc, m.xpos, m.v, m.rest, m.mass, m.gforce, m.ext, m.mask, td.inst_mu, td.inst_lam);
+  else
+    hipLaunchKernelGGL((k_ie_assemble<R, 1>), dim3(nb_of(d.n)), dim3(kBlock), 0, st, p, a, d, m.tets, m.dmi, m.vol,
+                       m.inc_off, m.inc, m.xpos, m.v, m.rest, m.mass, m.gforce, m.ext, m.mask, td.inst_mu, td.inst_lam);
+}
+
+template <typename R> void IeLaunch<R>::start(const IeDev &d, hipStream_t st) {
+  hipLaunchKernelGGL(k_ie_start<R>, dim3(nb_of(d.n)), dim3(kBlock), 0, st, d);
+  hipLaunchKernelGGL(k_ie_start_ctrl, dim3(1), dim3(1024), 0, st, d, nb_of(d.n));
+}
+
+template <typename R> void IeLaunch<R>::spmv(const IeDev &d, hipStream_t st) {
+  hipLaunchKernelGGL(k_ie_spmv<R>, dim3(nb_of(d.n)), dim3(kBlock), 0, st, d);
+}
+template <typename R> void IeLaunch<R>::alpha(const IeDev &d, hipStream_t st) {
+  hipLaunchKernelGGL(k_ie_alpha, dim3(1), dim3(1024), 0, st, d, nb_of(d.n));
+}
+template <typename R> void IeLaunch<R>::update(const IeDev &d, hipStream_t st) {
+  hipLaunchKernelGGL(k_ie_update, dim3(nb_of(d.n)), dim3(kBlock), 0, st, d);
+}
+template <typename R> void IeLaunch<R>::beta(const IeDev &d, hipStream_t st) {
+  hipLaunchKernelGGL(k_ie_beta, dim3(1), dim3(1024), 0, st, d, nb_of(d.n));
+}
+
+template <typename R>
+void IeLaunch<R>::finish(const DevMesh<R> &m, const DevParams &p, float h, const IeDev &d, hipStream_t st) {
+  hipLaunchKernelGGL(k_ie_finish<R>, dim3(nb_of(d.n)), dim3(kBlock), 0, st, p, h, d, m.xpos, m.v);
+}
+
+template struct IeLaunch<float>;
+template struct IeLaunch<double>;
+
+}  // namespace pbs

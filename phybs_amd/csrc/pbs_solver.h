@@ -15,6 +15,8 @@
 namespace pbs {
 struct LearnState;  // pbs_learn.h
 void learn_release(LearnState *L);
+struct ImplicitState;  // pbs_implicit.h
+void implicit_release(ImplicitState *S);
 }  // namespace pbs
 
 using namespace pbs;
@@ -25,9 +27,10 @@ using namespace pbs;
     if (e_ != hipSuccess) return set_error(PBS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-enum KernelId { K_TILE = 0, K_NODE, K_CONTROL, K_INIT, K_FINAL, K_LEARN, K_COUNT };
-static const char *const kKernelNames[K_COUNT] = {"k_tile", "k_node", "k_control", "k_step_init", "k_step_final",
-                                                  "k_warm_start"};
+enum KernelId { K_TILE = 0, K_NODE, K_CONTROL, K_INIT, K_FINAL, K_LEARN, K_IE_ASSEMBLE, K_IE_SPMV, K_IE_UPDATE, K_COUNT };
+static const char *const kKernelNames[K_COUNT] = {"k_tile",       "k_node",        "k_control",
+                                                  "k_step_init",  "k_step_final",  "k_warm_start",
+                                                  "k_ie_assemble", "k_ie_spmv",    "k_ie_update"};
 
 struct pbs_solver {
   int nn = 0, nt = 0;
@@ -72,6 +75,7 @@ struct pbs_solver {
   double kt_ms[K_COUNT] = {0};
   long kt_n[K_COUNT] = {0};
   pbs::LearnState *learn = nullptr;  // learned warm start (created on first use)
+  pbs::ImplicitState *implicit = nullptr;  // backward-Euler matrix and CG buffers (first use)
 };
 
 template <typename R> inline DevMesh<R> &mesh(pbs_solver *s);

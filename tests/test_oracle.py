@@ -249,6 +249,73 @@ def test_hessian_diag_fd(material):
 
 
 # ---------------------------------------------------------------------------------------------
+# backward-Euler system (the reference's GPU implicit path)
+def implicit_case(material=O.MAT_NEOHOOKEAN, seed=3):
+    mu, lam = O.lame(E_, NU)
+    rest, tets = M.bar(2, 2, 3, 0.05)
+    mask = M.clamp_x0(rest)
+    pre = O.precompute(rest, tets, mask)
+    rng = np.random.default_rng(seed)
+    x = rest.astype(np.float64) + 0.006 * rng.standard_normal(rest.shape)
+    v = 0.1 * rng.standard_normal(rest.shape)
+    return mu, lam, rest, tets, mask, pre, x, v
+
+
+def dense(rowptr, col, val, n):
+    A = np.zeros((n, n))
+    for r in range(n):
+        A[r, col[rowptr[r]:rowptr[r + 1]]] = val[rowptr[r]:rowptr[r + 1]]
+    return A
+
+
+@pytest.mark.parametrize("material", [O.MAT_NEOHOOKEAN, O.MAT_STRETCH])
+def test_implicit_stiffness_is_minus_force_jacobian(material):
+    """K = -df/dx: central differences of the right-hand side b = h f + m v (free rows)."""
+    mu, lam, rest, tets, mask, pre, x, v = implicit_case(material)
+    h = 0.03
+    rowptr, col, kval, aval, b0 = O.implicit_system(rest, tets, mask, pre, mu, lam, x, v, h, material=material)
+    K = dense(rowptr, col, kval, 3 * len(rest))
+    eps = 1e-6
+    free = np.repeat(mask == 0, 3)
+    for j in range(3 * len(rest)):
+        xp, xm = x.copy().reshape(-1), x.copy().reshape(-1)
+        xp[j] += eps
+        xm[j] -= eps
+        bp = O.implicit_system(rest, tets, mask, pre, mu, lam, xp, v, h, material=material)[4]
+        bm = O.implicit_system(rest, tets, mask, pre, mu, lam, xm, v, h, material=material)[4]
+        fd = -(bp - bm) / (2 * eps * h)
+        assert np.allclose(K[free, j], fd[free], rtol=1e-5, atol=1e-6 * np.abs(K).max()), j
+    assert np.allclose(K, K.T, atol=1e-9 * np.abs(K).max())
+    for d in range(3):  # rigid translations are in the null space
+        t = np.zeros((len(rest), 3))
+        t[:, d] = 1.0
+        assert np.abs(K @ t.reshape(-1)).max() <= 1e-9 * np.abs(K).max()
+
+
+def test_implicit_system_matrix_and_rhs():
+    mu, lam, rest, tets, mask, pre, x, v = implicit_case()
+    h, al, be = (float(np.float32(c)) for c in (0.03, 0.02, 0.02))  # the oracle takes FP32 constants
+    rowptr, col, kval, aval, b = O.implicit_system(rest, tets, mask, pre, mu, lam, x, v, h, al, be)
+    n3 = 3 * len(rest)
+    K = dense(rowptr, col, kval, n3)
+    A = dense(rowptr, col, aval, n3)
+    m3 = np.repeat(pre["mass"].astype(np.float64), 3)
+    assert np.allclose(A, (h + be) * h * K + np.diag((1 + al * h) * m3), rtol=1e-12, atol=1e-12)
+    # b - h f_elastic = h m g + m v on free nodes (forces from the SVD route = closed-form forces)
+    f, _ = O.tet_forces(x, tets, pre, mu, lam, O.MAT_NEOHOOKEAN, O.CLAMP_NONE, 64)
+    fe = np.zeros_like(x)
+    np.add.at(fe, tets.reshape(-1), f.reshape(-1, 3))
+    want = h * (fe + pre["gforce"]) + pre["mass"][:, None] * v
+    want[mask != 0] = h * 1e8 * (rest[mask != 0] - x[mask != 0]) + 1e8 * v[mask != 0]
+    assert np.allclose(b.reshape(-1, 3), want, rtol=1e-9, atol=1e-9 * np.abs(want).max())
+    # ext enters the free nodes' forces only
+    ext = np.ones_like(x)
+    b2 = O.implicit_system(rest, tets, mask, pre, mu, lam, x, v, h, al, be, ext=ext)[4].reshape(-1, 3)
+    assert np.allclose((b2 - b.reshape(-1, 3))[mask == 0], h)
+    assert np.array_equal(b2[mask != 0], b.reshape(-1, 3)[mask != 0])
+
+
+# ---------------------------------------------------------------------------------------------
 # descent loop
 def test_step_mass_only_closed_form():
     """With E = 0 the incremental potential is 1/2 m |x - x~|^2 / h^2 - m g.x: minimiser
