@@ -193,6 +193,65 @@ def warm_start_report(P, args, params, rest, tets, mask, x0, dims, plain_sps, pl
     return out
 
 
+def implicit_report(P, rest, tets, mask, material, grav, steps=3):
+    """Backward-Euler steps (the reference's GPU implicit path) on the benchmark bar in FP32 from
+    rest under gravity: steps/s, CG iterations, and the SpMV kernel against HBM bandwidth.  SpMV
+    algorithmic bytes per launch: every stored 3x3 block once (9 FP32 values + a 4-byte column
+    index) plus 4 vectors of 3 doubles per node (z and p_{k-1} read, p_k and A p_k written)."""
+    prm = P.default_params(dt=1.0 / 30.0, grav=grav, material=material)
+    s = P.Solver(rest, tets, mask, prm)
+    ip = P.implicit_params()
+    s.implicit_step(params=ip)  # warm-up: builds the matrix layout and the CG graphs
+    s.set_profiling(1)
+    s.reset_kernel_times()
+    t0 = time.perf_counter()
+    st = [s.implicit_step(params=ip) for _ in range(steps)]
+    el = time.perf_counter() - t0
+    kt = s.kernel_times()
+    s.close()
+    t32 = np.ascontiguousarray(tets, np.int32)
+    nblocks = P.lib().pbs_csr_pattern(len(rest), len(t32), t32.ctypes.data, None, None, None, None) // 9
+    nn = len(rest)
+    spmv_bytes = nblocks * (9 * 4 + 4) + 4 * 24 * nn
+    upd_bytes = 8 * 24 * nn
+    spmv_ms, upd_ms = kt["k_ie_spmv"][0], kt["k_ie_update"][0]
+    ach = spmv_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else 0.0
+    return {
+        "steps": steps, "steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps, "dtype": "f32 matrix, f64 CG",
+        "cg_tolerance": ip.cg_tolerance, "cg_iterations_per_step": float(np.mean([q.cg_iterations for q in st])),
+        "converged_steps": int(sum(q.converged for q in st)),
+        "kernel_avg_ms": {k: v[0] for k, v in kt.items() if k.startswith("k_ie")},
+        "spmv_roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": spmv_bytes,
+                          "blocks": nblocks},
+        "update_achieved_gbs": upd_bytes / (upd_ms * 1e-3) / 1e9 if upd_ms > 0 else 0.0,
+    }
+
+
+def implicit_cpu_baseline(P, grav):
+    """CPU reference point for one backward-Euler step on a 12x12x50 bar (43,200 tets): the
+    oracle's FP64 system assembly plus scipy's sparse direct solve (the reference factorises with
+    a sparse Cholesky), timed on the host."""
+    import scipy.sparse as sps
+    import scipy.sparse.linalg as spl
+
+    from oracle import pyoracle as O
+    rest, tets = P.mesh_bar(12, 12, 50, 0.05)
+    mask = (rest[:, 0] == 0).astype(np.int32)
+    mu, lam = O.lame(1e5, 0.45)
+    pre = O.precompute(rest, tets, mask, 1000.0, grav, 1e8)
+    x = rest.astype(np.float64)
+    v = np.zeros_like(x)
+    t0 = time.perf_counter()
+    rowptr, col, _, aval, b = O.implicit_system(rest, tets, mask, pre, mu, lam, x, v, 1.0 / 30.0)
+    n3 = 3 * len(rest)
+    spl.spsolve(sps.csr_matrix((aval, col, rowptr), shape=(n3, n3)).tocsc(), b)
+    el = time.perf_counter() - t0
+    return {"value": 1.0 / el, "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": "one backward-Euler step of a 12x12x50 bar (43,200 tets): oracle FP64 assembly + "
+                      "scipy sparse direct solve"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +265,7 @@ def main():
     ap.add_argument("--max-iterations", type=int, default=40000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
+    ap.add_argument("--no-implicit", action="store_true", help="skip the backward-Euler report")
     ap.add_argument("--train-steps", type=int, default=6, help="training-trajectory steps on the benchmark bar")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) for counter calibration")
@@ -311,6 +371,10 @@ def main():
     if relg and not args.no_learned and not batch:
         out["learned_warm_start"] = warm_start_report(P, args, params, rest, tets, mask, x0, (nx, ny, nz),
                                                       steps_per_s, iters / args.steps)
+    if not args.no_implicit and not batch:
+        out["implicit_euler"] = implicit_report(P, rest, tets, mask, material, grav)
+        if not args.no_cpu_baseline:
+            out["implicit_euler"]["cpu_baseline"] = implicit_cpu_baseline(P, grav)
     if not args.no_cpu_baseline and batch:
         out["cpu_baseline"] = cpu_baseline_batch(inst, fixed_iters, grav)
     elif not args.no_cpu_baseline:
