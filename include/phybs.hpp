@@ -109,11 +109,25 @@ class CUDAInterface {
   }
   void reset() { solver_->reset(); }
   void updateNodePositions(const float *hostNode) { solver_->set_positions(hostNode); }
+  // The implicit path as fvm.cpp:179-183 drives it: computeInnerforces(), then
+  // computeGlobalStiffnessMatrix(), then doBackEuler(nodes).  pbs_implicit_step evaluates the
+  // forces and the stiffness matrix itself, so the first two calls only mark the step as started.
+  void computeInnerforces() { forces_ready_ = true; }
+  void computeGlobalStiffnessMatrix() { stiffness_ready_ = true; }
+  void doBackEuler(float *hostNode) {
+    if (!forces_ready_ || !stiffness_ready_)
+      throw Error(PBS_ERR_STATE, "doBackEuler: call computeInnerforces and computeGlobalStiffnessMatrix first");
+    pbs_implicit_params ip;
+    check(pbs_implicit_default_params(&ip));
+    check(pbs_implicit_step(solver_->handle(), &ip, nullptr, hostNode, nullptr));
+    forces_ready_ = stiffness_ready_ = false;
+  }
   Solver &solver() { return *solver_; }
 
  private:
   int nn_;
   std::unique_ptr<Solver> solver_;
+  bool forces_ready_ = false, stiffness_ready_ = false;
 };
 
 // The CPU descent class of the reference, on the GPU.  Each doDescentOpt(iterations) takes the

@@ -2,6 +2,8 @@
 // (fvm.cpp:30, 44-47, 175, 187, 216-217) on a 4x4x16 bar, and prints the resulting positions.
 //   compat_demo cuda <steps>     CUDAInterface: doDescentOpt(h, 96, ext, nodes) per step
 //   compat_demo descent <steps>  DescentOptimizeT over a minimal mesh type: doDescentOpt(96)
+//   compat_demo backeuler <steps> CUDAInterface: computeInnerforces, computeGlobalStiffnessMatrix,
+//                                 doBackEuler(nodes) per step (fvm.cpp:179-183)
 // Output: one line per node, "x y z" with 9 significant digits.
 #include <cstdio>
 #include <cstdlib>
@@ -46,10 +48,11 @@ struct DemoMaterial {};
 
 int main(int argc, char **argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: compat_demo cuda|descent <steps>\n");
+    std::fprintf(stderr, "usage: compat_demo cuda|descent|backeuler <steps>\n");
     return 2;
   }
   const bool cuda = std::strcmp(argv[1], "cuda") == 0;
+  const bool backeuler = std::strcmp(argv[1], "backeuler") == 0;
   const int steps = std::atoi(argv[2]);
   const int nx = 4, ny = 4, nz = 16;
   const float edge = 0.05f;
@@ -63,7 +66,16 @@ int main(int argc, char **argv) {
   for (int i = 0; i < nn; ++i) mask[i] = rest[3 * i] == 0.0f;
   std::vector<float> out(x0);
   try {
-    if (cuda) {
+    if (backeuler) {
+      phybs::CUDAInterface gpu(nn, x0.data(), rest.data(), mask.data(), nt, tets.data(), 1e5f, 0.45f, 1000.0f, 0,
+                               0, nullptr, nullptr, nullptr, nullptr, nullptr);
+      gpu.updateNodePositions(x0.data());
+      for (int s = 0; s < steps; ++s) {
+        gpu.computeInnerforces();
+        gpu.computeGlobalStiffnessMatrix();
+        gpu.doBackEuler(out.data());
+      }
+    } else if (cuda) {
       phybs::CUDAInterface gpu(nn, x0.data(), rest.data(), mask.data(), nt, tets.data(), 1e5f, 0.45f, 1000.0f, 0,
                                0, nullptr, nullptr, nullptr, nullptr, nullptr);
       gpu.updateNodePositions(x0.data());

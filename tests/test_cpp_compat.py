@@ -58,6 +58,18 @@ def test_cuda_interface_shim_matches_c_abi():
 
 
 @pytest.mark.gpu
+def test_back_euler_shim_matches_c_abi():
+    rest, tets, mask, x0 = bar()
+    got = demo_positions("backeuler", 3)
+    s = P.Solver(rest, tets, mask, P.default_params(youngs=1e5, nu=0.45, density=1000.0))
+    s.set_positions(x0)
+    for _ in range(3):
+        _, ref = s.implicit_step(want_positions=True)
+    s.close()
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.gpu
 def test_descent_optimize_shim_matches_c_abi():
     rest, tets, mask, x0 = bar()
     got = demo_positions("descent", 3)
