@@ -228,6 +228,39 @@ def implicit_report(P, rest, tets, mask, material, grav, steps=3):
     }
 
 
+def pd_report(P, rest, tets, mask, grav, steps=2):
+    """Projective-dynamics steps (5 local/global rounds each) on the benchmark bar in FP32 from rest
+    under gravity: steps/s, CG iterations and the local step against HBM bandwidth.  Local-step
+    algorithmic bytes per launch: 56 per tet (ids, Dm^-1, volume) + 72 per node (the FP64 iterate
+    and momentum term read, the right-hand side written)."""
+    prm = P.default_params(dt=1.0 / 30.0, grav=grav)
+    s = P.Solver(rest, tets, mask, prm)
+    pp = P.pd_params()
+    s.pd_step(params=pp)  # warm-up: matrix layout and graphs
+    s.set_profiling(1)
+    s.reset_kernel_times()
+    t0 = time.perf_counter()
+    st = [s.pd_step(params=pp) for _ in range(steps)]
+    el = time.perf_counter() - t0
+    kt = s.kernel_times()
+    s.close()
+    nt, nn = len(tets), len(rest)
+    loc_bytes = 56 * nt + 72 * nn
+    loc_ms, spmv_ms = kt["k_pd_local"][0], kt["k_ie_spmv"][0]
+    t32 = np.ascontiguousarray(tets, np.int32)
+    nblocks = P.lib().pbs_csr_pattern(nn, len(t32), t32.ctypes.data, None, None, None, None) // 9
+    spmv_bytes = nblocks * 8 + 4 * 24 * nn
+    return {
+        "steps": steps, "steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps, "dtype": "f32 matrix, f64 CG",
+        "rounds_per_step": pp.iterations, "cg_tolerance": pp.cg_tolerance,
+        "cg_iterations_per_step": float(np.mean([q.cg_iterations for q in st])),
+        "converged_steps": int(sum(q.converged for q in st)),
+        "kernel_avg_ms": {"k_pd_local": loc_ms, "k_ie_spmv (scalar)": spmv_ms, "k_ie_update": kt["k_ie_update"][0]},
+        "local_step_achieved_gbs": loc_bytes / (loc_ms * 1e-3) / 1e9 if loc_ms > 0 else 0.0,
+        "spmv_achieved_gbs": spmv_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else 0.0,
+    }
+
+
 def implicit_cpu_baseline(P, grav):
     """CPU reference point for one backward-Euler step on a 12x12x50 bar (43,200 tets): the
     oracle's FP64 system assembly plus scipy's sparse direct solve (the reference factorises with
@@ -373,6 +406,7 @@ def main():
                                                       steps_per_s, iters / args.steps)
     if not args.no_implicit and not batch:
         out["implicit_euler"] = implicit_report(P, rest, tets, mask, material, grav)
+        out["projective_dynamics"] = pd_report(P, rest, tets, mask, grav)
         if not args.no_cpu_baseline:
             out["implicit_euler"]["cpu_baseline"] = implicit_cpu_baseline(P, grav)
     if not args.no_cpu_baseline and batch:

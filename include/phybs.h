@@ -204,6 +204,32 @@ int pbs_implicit_step(pbs_solver *s, const pbs_implicit_params *p, const float *
  * CUDAInterface.cu:234-271; pattern = pbs_csr_pattern's rowptr/col, nnz values). */
 int pbs_stiffness_matrix(pbs_solver *s, int64_t nnz, double *csr_val);
 
+/* ---- projective dynamics (ProjDynamic::doProjDynamics, ProjDynamic.cpp:442-479) ---- */
+typedef struct pbs_pd_params {
+  float stiffness;       /* 1e6: constraint weight per unit volume (m_stiffness, ProjDynamic.cpp:4) */
+  int iterations;        /* 5 local/global rounds (m_iterations) */
+  float damping;         /* 0.99: v = damping (x' - x) / dt (:477) */
+  double cg_tolerance;   /* 1e-10: each global solve, in the Jacobi-scaled norm, whose |D^-1/2 b|
+                            the fixed nodes' 1e8-mass rows dominate (the reference factorises
+                            once with Pardiso LDLT) */
+  int cg_max_iterations; /* 10000 per global solve */
+} pbs_pd_params;
+typedef struct pbs_pd_stats {
+  int cg_iterations;     /* summed over the rounds */
+  int converged;         /* every global solve met the tolerance */
+  float gpu_ms;
+} pbs_pd_stats;
+int pbs_pd_default_params(pbs_pd_params *p);
+/* One projective-dynamics step with the handle's dt, mass and gravity: s = M (x + dt v)/dt^2 +
+ * f_ext (free nodes; f_ext = gravity + ext, ext may be NULL), then `iterations` rounds of the
+ * local step (per-tet rotation R of F, see pbs_pd_project) and the global solve
+ * [M/dt^2 + sum_k w_k Ac_k^T Ac_k] x = s + projections; v' = damping (x' - x)/dt, x = x'. */
+int pbs_pd_step(pbs_solver *s, const pbs_pd_params *p, const float *ext, float *out, pbs_pd_stats *st);
+/* Local step alone (projectLocalConstraints2, ProjDynamic.cpp:223-308): for positions (3N AoS),
+ * proj (3N AoS) = sum over each node's tets k of w_k R_k c_i, w_k = stiffness * vol_k, c_i the
+ * node's shape vector and R_k = U V^T of F_k with the reference's sign fixes (:359-368). */
+int pbs_pd_project(pbs_solver *s, float stiffness, const float *positions, double *proj);
+
 /* ---- host mesh utilities (no GPU work) ---- */
 /* Bar of nx*ny*nz cubes of size edge, 6 tets per cube (Freudenthal split, det(Dm) > 0).
  * nodes: 3*(nx+1)(ny+1)(nz+1) floats, tets: 24*nx*ny*nz ints; node id = i + (nx+1)(j + (ny+1)k). */

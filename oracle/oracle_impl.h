@@ -412,6 +412,99 @@ void FN(orc_implicit_system)(int nn, const float *rest, int nt, const int *tets,
   free(f);
 }
 
+/* Projection rotation of projective dynamics (ProjDynamic.cpp:359-368): the SVD F = U S V^T with
+ * singular values in decreasing order (the reference calls Eigen::JacobiSVD; here the one-sided
+ * Jacobi of svd3_core in FP64 with U = F V S^-1), then U's column 0 negated if det U < 0 and V's
+ * column 0 negated if det V < 0, R = U V^T; rounded to R. */
+void FN(orc_pd_rotation)(const R *F, R *Rot) {
+  double Fd[9], S[3], V[9], U[9];
+  for (int i = 0; i < 9; ++i) Fd[i] = (double)F[i];
+  svd3_core(Fd, S, V);
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      double a = Fd[3 * r] * V[c] + Fd[3 * r + 1] * V[3 + c] + Fd[3 * r + 2] * V[6 + c];
+      U[3 * r + c] = S[c] > 1e-300 ? a / S[c] : 0.0;
+    }
+  if (!(S[2] > 1e-300)) { /* complete a rank-2 U by the cross product of its first columns */
+    U[2] = U[3] * U[7] - U[6] * U[4];
+    U[5] = U[6] * U[1] - U[0] * U[7];
+    U[8] = U[0] * U[4] - U[3] * U[1];
+  }
+  double du = U[0] * (U[4] * U[8] - U[7] * U[5]) + U[3] * (U[7] * U[2] - U[1] * U[8]) + U[6] * (U[1] * U[5] - U[4] * U[2]);
+  double dv = V[0] * (V[4] * V[8] - V[7] * V[5]) + V[3] * (V[7] * V[2] - V[1] * V[8]) + V[6] * (V[1] * V[5] - V[4] * V[2]);
+  if (du < 0) { U[0] = -U[0]; U[3] = -U[3]; U[6] = -U[6]; }
+  if (dv < 0) { V[0] = -V[0]; V[3] = -V[3]; V[6] = -V[6]; }
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      Rot[3 * r + c] = (R)(U[3 * r] * V[3 * c] + U[3 * r + 1] * V[3 * c + 1] + U[3 * r + 2] * V[3 * c + 2]);
+}
+
+/* Local step of projective dynamics (projectLocalConstraints2, ProjDynamic.cpp:223-308): per tet
+ * c = w Ac^T vec(R) with w = stiffness * vol, i.e. corner i gets w R c_i (c_0 = minus the column sums
+ * of Dm^-1, c_i = row i-1); out = s + the corner projections summed per node in ascending tet
+ * order, starting from s (s may be NULL: zero).  x, s, out: 3N AoS. */
+void FN(orc_pd_project)(int nn, int nt, const int *tets, const R *x, const float *dminv, const float *vol,
+                        float stiffness, const R *s, R *out) {
+  R *proj = (R *)malloc(sizeof(R) * 12 * (size_t)(nt > 0 ? nt : 1));
+#pragma omp parallel for schedule(static)
+  for (int t = 0; t < nt; ++t) {
+    R F[9], Rot[9];
+    FN(def_grad)(tets + 4 * t, x, dminv + 9 * t, F);
+    FN(orc_pd_rotation)(F, Rot);
+    const float *D = dminv + 9 * t;
+    R cv[4][3];
+    for (int j = 0; j < 3; ++j) {
+      cv[0][j] = -(R)D[j] - (R)D[3 + j] - (R)D[6 + j];
+      cv[1][j] = (R)D[j];
+      cv[2][j] = (R)D[3 + j];
+      cv[3][j] = (R)D[6 + j];
+    }
+    const R w = (R)stiffness * (R)vol[t];
+    for (int i = 0; i < 4; ++i)
+      for (int r = 0; r < 3; ++r)
+        proj[12 * (size_t)t + 3 * i + r] = (cv[i][0] * Rot[3 * r] + cv[i][1] * Rot[3 * r + 1] + cv[i][2] * Rot[3 * r + 2]) * w;
+  }
+  for (int i = 0; i < 3 * nn; ++i) out[i] = s ? s[i] : (R)0;
+  for (int t = 0; t < nt; ++t)
+    for (int i = 0; i < 4; ++i)
+      for (int r = 0; r < 3; ++r) out[3 * tets[4 * t + i] + r] += proj[12 * (size_t)t + 3 * i + r];
+  free(proj);
+}
+
+/* Global matrix of projective dynamics (buildGlobalSolverMatrix, ProjDynamic.cpp:46-96): the
+ * explicit 9x12 Ac (rows 3a..3a+2 of column block i hold c_i), w Ac^T Ac added per tet via kid, and
+ * m / t^2 on the diagonal; values on the CSR pattern of orc_csr_pattern. */
+void FN(orc_pd_matrix)(int nn, int nt, const int *tets, const float *dminv, const float *vol, const float *mass,
+                       float stiffness, float t_f, long nnz, const int *kid, const int *diag_idx, double *gval) {
+  R *g = (R *)calloc((size_t)(nnz > 0 ? nnz : 1), sizeof(R));
+  const R t = (R)t_f, t2inv = (R)1 / (t * t);
+  for (int n = 0; n < nn; ++n)
+    for (int d = 0; d < 3; ++d) g[diag_idx[3 * n + d]] += (R)mass[n] * t2inv;
+  for (int k = 0; k < nt; ++k) {
+    const float *D = dminv + 9 * k;
+    R Ac[9][12];
+    memset(Ac, 0, sizeof(Ac));
+    for (int a = 0; a < 3; ++a)
+      for (int r = 0; r < 3; ++r) {
+        Ac[3 * a + r][a] = -(R)D[r] - (R)D[3 + r] - (R)D[6 + r];
+        Ac[3 * a + r][3 + a] = (R)D[r];
+        Ac[3 * a + r][6 + a] = (R)D[3 + r];
+        Ac[3 * a + r][9 + a] = (R)D[6 + r];
+      }
+    const R w = (R)stiffness * (R)vol[k];
+    for (int fi = 0; fi < 4; ++fi)
+      for (int fj = 0; fj < 3; ++fj)
+        for (int ni = 0; ni < 4; ++ni)
+          for (int nj = 0; nj < 3; ++nj) {
+            R acc = 0;
+            for (int q = 0; q < 9; ++q) acc += Ac[q][3 * fi + fj] * Ac[q][3 * ni + nj];
+            g[kid[48 * (size_t)k + 4 * (3 * fi + fj) + ni] + nj] += w * acc;
+          }
+  }
+  for (long i = 0; i < nnz; ++i) gval[i] = (double)g[i];
+  free(g);
+}
+
 /* ------------------------------------------------------------------------------------------
  * Descent solver state in precision R.
  * ------------------------------------------------------------------------------------------ */

@@ -25,6 +25,13 @@
  *                       forces come from the SVD route, pinned against the closed-form forces),
  *                       translation null space K t = 0, symmetry, the system matrix against
  *                       its definition (test_oracle.py::test_implicit_*)
+ *   orc_pd_rotation     orthogonal with det +1; the polar factor (scipy.linalg.polar) when det F > 0;
+ *                       numpy's SVD with the reference's sign fixes when det F < 0
+ *                       (test_oracle.py::test_pd_rotation_*)
+ *   orc_pd_project      the gradient of sum_k w_k <F_k(x), R_k> with R_k frozen, by central
+ *                       differences (test_oracle.py::test_pd_project_fd)
+ *   orc_pd_matrix       the Hessian of sum_k w_k |F_k(x)|^2 / 2 by differences of its gradient,
+ *                       plus M / t^2 (test_oracle.py::test_pd_matrix_fd)
  *   orc_csr_pattern     brute force from the tet list: row pattern, diagonal index, kID
  *                       (test_mesh.py::test_csr_pattern_bit_exact)
  *   orc_incidence       brute force (test_mesh.py::test_incidence_bit_exact)

@@ -127,6 +127,20 @@ void orc_implicit_system_f64(int nn, const float *rest, int nt, const int *tets,
                              const double *ext, float h, float alpha, float beta, long nnz, const int *kid,
                              const int *diag_idx, double *kval, double *aval, double *b);
 
+/* ---- projective dynamics (ProjDynamic.cpp:46-96, 223-308, 359-368) ----
+ * rotation: R = U V^T with the reference's sign fixes; project: out = s + per-node sums of
+ * w R c_i (s may be NULL), x/s/out 3N AoS; matrix: M/t^2 + sum w Ac^T Ac on orc_csr_pattern. */
+void orc_pd_rotation_f32(const float *F, float *Rot);
+void orc_pd_rotation_f64(const double *F, double *Rot);
+void orc_pd_project_f32(int nn, int nt, const int *tets, const float *x, const float *dminv, const float *vol,
+                        float stiffness, const float *s, float *out);
+void orc_pd_project_f64(int nn, int nt, const int *tets, const double *x, const float *dminv, const float *vol,
+                        float stiffness, const double *s, double *out);
+void orc_pd_matrix_f32(int nn, int nt, const int *tets, const float *dminv, const float *vol, const float *mass,
+                       float stiffness, float t, long nnz, const int *kid, const int *diag_idx, double *gval);
+void orc_pd_matrix_f64(int nn, int nt, const int *tets, const float *dminv, const float *vol, const float *mass,
+                       float stiffness, float t, long nnz, const int *kid, const int *diag_idx, double *gval);
+
 /* ---- adjacency (IsotropicMaterial.cpp:554-644, IsotropicMaterial.h:120-138; ProjDynamic.cpp:25-45) ---- */
 /* Pattern of the 3N x 3N stiffness matrix: returns nnz; rowptr (3N+1) and colidx (nnz) are
  * written when non-NULL (call once with NULLs to size).  diag_idx (3N) and kid (48T) as the

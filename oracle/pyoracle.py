@@ -78,6 +78,11 @@ def lib():
         L.orc_csr_pattern.restype = C.c_long
         L.orc_incidence.argtypes = [C.c_int, C.c_int, P, P, P]
         for sfx in ("f32", "f64"):
+            getattr(L, "orc_pd_rotation_" + sfx).argtypes = [P, P]
+            getattr(L, "orc_pd_project_" + sfx).argtypes = [C.c_int, C.c_int, P, P, P, P, C.c_float, P, P]
+            getattr(L, "orc_pd_matrix_" + sfx).argtypes = [C.c_int, C.c_int, P, P, P, P, C.c_float, C.c_float,
+                                                           C.c_long, P, P, P]
+        for sfx in ("f32", "f64"):
             getattr(L, "orc_implicit_system_" + sfx).argtypes = [
                 C.c_int, P, C.c_int, P, P, P, P, P, P, C.c_float, C.c_float, C.c_int, P, P, P,
                 C.c_float, C.c_float, C.c_float, C.c_long, P, P, P, P, P]
@@ -213,6 +218,36 @@ def implicit_system(rest, tets, mask, pre, mu, lam, x, v, h=0.03, alpha=0.02, be
        _p(np.ascontiguousarray(pre["gforce"], np.float32)), mu, lam, material, _p(x), _p(v), _p(e), h, alpha, beta,
        nnz, _p(kid), _p(diag), _p(kval), _p(aval), _p(b))
     return rowptr, col, kval, aval, b
+
+
+def pd_rotation(F, precision=64):
+    dt = np.float32 if precision == 32 else np.float64
+    F = np.ascontiguousarray(F, dt).reshape(9)
+    out = np.zeros(9, dt)
+    (lib().orc_pd_rotation_f32 if precision == 32 else lib().orc_pd_rotation_f64)(_p(F), _p(out))
+    return out.reshape(3, 3)
+
+
+def pd_project(x, tets, pre, stiffness=1e6, s=None, precision=64):
+    dt = np.float32 if precision == 32 else np.float64
+    x = np.ascontiguousarray(x, dt).reshape(-1, 3)
+    tets = np.ascontiguousarray(tets, np.int32).reshape(-1, 4)
+    sa = None if s is None else np.ascontiguousarray(s, dt).reshape(-1)
+    out = np.zeros_like(x)
+    fn = lib().orc_pd_project_f32 if precision == 32 else lib().orc_pd_project_f64
+    fn(len(x), len(tets), _p(tets), _p(x), _p(pre["dminv"]), _p(pre["vol"]), stiffness, _p(sa), _p(out))
+    return out
+
+
+def pd_matrix(nn, tets, pre, stiffness=1e6, t=0.03, precision=64):
+    """(rowptr, col, values) of the projective-dynamics global matrix."""
+    tets = np.ascontiguousarray(tets, np.int32).reshape(-1, 4)
+    rowptr, col, diag, kid = csr_pattern(nn, tets)
+    g = np.zeros(len(col))
+    fn = lib().orc_pd_matrix_f32 if precision == 32 else lib().orc_pd_matrix_f64
+    fn(nn, len(tets), _p(tets), _p(pre["dminv"]), _p(pre["vol"]), _p(pre["mass"]), stiffness, t, len(col), _p(kid),
+       _p(diag), _p(g))
+    return rowptr, col, g
 
 
 class Solver:

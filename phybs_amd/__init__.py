@@ -66,6 +66,25 @@ class ImplicitStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class PdParams(C.Structure):
+    _fields_ = [
+        ("stiffness", C.c_float), ("iterations", C.c_int), ("damping", C.c_float), ("cg_tolerance", C.c_double),
+        ("cg_max_iterations", C.c_int),
+    ]
+
+
+class PdStats(C.Structure):
+    _fields_ = [("cg_iterations", C.c_int), ("converged", C.c_int), ("gpu_ms", C.c_float)]
+
+
+def pd_params(**kw):
+    p = PdParams()
+    _check(lib().pbs_pd_default_params(C.byref(p)))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
 def implicit_params(**kw):
     p = ImplicitParams()
     _check(lib().pbs_implicit_default_params(C.byref(p)))
@@ -115,6 +134,9 @@ SIGNATURES = {
     "pbs_implicit_default_params": (C.c_int, [_P]),
     "pbs_implicit_step": (C.c_int, [_P, _P, _P, _P, _P]),
     "pbs_stiffness_matrix": (C.c_int, [_P, C.c_int64, _P]),
+    "pbs_pd_default_params": (C.c_int, [_P]),
+    "pbs_pd_step": (C.c_int, [_P, _P, _P, _P, _P]),
+    "pbs_pd_project": (C.c_int, [_P, C.c_float, _P, _P]),
     "pbs_face_ring": (C.c_int64, [C.c_int, C.c_int, _P, _P, _P]),
     "pbs_pick_face": (C.c_int, [C.c_int, _P, C.c_int, _P, _P, _P, _P]),
     "pbs_drag_face_ring": (C.c_int, [C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_int, _P, _P]),
@@ -387,6 +409,22 @@ class Solver:
         out = np.zeros((self.nn, 3), np.float32) if want_positions else None
         _check(lib().pbs_implicit_step(self.h, C.byref(p), _p(e), _p(out), C.byref(st)))
         return (st, out) if want_positions else st
+
+    def pd_step(self, ext=None, params=None, want_positions=False):
+        """One projective-dynamics step; returns PdStats (and the new positions)."""
+        st = PdStats()
+        p = params if params is not None else pd_params()
+        e = None if ext is None else _f32(ext)
+        out = np.zeros((self.nn, 3), np.float32) if want_positions else None
+        _check(lib().pbs_pd_step(self.h, C.byref(p), _p(e), _p(out), C.byref(st)))
+        return (st, out) if want_positions else st
+
+    def pd_project(self, positions, stiffness=1e6):
+        """Local step alone: per-node sums of w_k R_k c_i (3N doubles, caller order)."""
+        pos = _f32(positions)
+        out = np.zeros((self.nn, 3))
+        _check(lib().pbs_pd_project(self.h, stiffness, _p(pos), _p(out)))
+        return out
 
     def stiffness_matrix(self, tets):
         """K = -df/dx at the current positions: (rowptr, col, values) in the reference's CSR layout."""

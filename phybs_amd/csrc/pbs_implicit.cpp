@@ -17,8 +17,10 @@ namespace pbs {
 
 void implicit_release(ImplicitState *S) {
   if (!S) return;
-  if (S->graph) (void)hipGraphExecDestroy(S->graph);
-  if (S->tgraph) (void)hipGraphExecDestroy(S->tgraph);
+  for (CgGraphs *G : {&S->cg_ie, &S->cg_pd}) {
+    if (G->graph) (void)hipGraphExecDestroy(G->graph);
+    if (G->tgraph) (void)hipGraphExecDestroy(G->tgraph);
+  }
   for (auto e : S->ev) (void)hipEventDestroy(e);
   for (void *q : S->allocs) (void)hipFree(q);
   delete S;
@@ -129,16 +131,15 @@ int implicit_get(pbs_solver *s, ImplicitState **out) {
   return PBS_OK;
 }
 
-// One graph holds kGraphIters CG iterations; the timed variant brackets each SpMV and each
-// vector update with external event records.
-template <typename R> int cg_graph(pbs_solver *s, ImplicitState *S, int timed) {
-  hipGraphExec_t &g = timed ? S->tgraph : S->graph;
-  const double key[2] = {S->dev.tol2, (double)S->dev.maxit};
-  if (g && std::memcmp(key, S->graph_key, sizeof(key)) == 0) return PBS_OK;
-  if (S->graph) (void)hipGraphExecDestroy(S->graph);
-  if (S->tgraph) (void)hipGraphExecDestroy(S->tgraph);
-  S->graph = S->tgraph = nullptr;
-  std::memcpy(S->graph_key, key, sizeof(key));
+// One graph holds kGraphIters CG iterations on the matrix of `dev`; the timed variant brackets
+// each SpMV and each vector update with external event records.
+template <typename R> int cg_graph(pbs_solver *s, ImplicitState *S, const IeDev &dev, CgGraphs &G, int timed) {
+  const double key[2] = {dev.tol2, (double)dev.maxit};
+  if ((timed ? G.tgraph : G.graph) && std::memcmp(key, G.key, sizeof(key)) == 0) return PBS_OK;
+  if (G.graph) (void)hipGraphExecDestroy(G.graph);
+  if (G.tgraph) (void)hipGraphExecDestroy(G.tgraph);
+  G.graph = G.tgraph = nullptr;
+  std::memcpy(G.key, key, sizeof(key));
   for (int t = 0; t < 2; ++t) {
     if (t == 1 && !timed) break;
     while (t == 1 && S->ev.size() < 4 * (size_t)kGraphIters + 2) {
@@ -150,13 +151,13 @@ template <typename R> int cg_graph(pbs_solver *s, ImplicitState *S, int timed) {
     hipError_t err = hipSuccess;
     for (int i = 0; i < kGraphIters && err == hipSuccess; ++i) {
       if (t) err = hipEventRecordWithFlags(S->ev[4 * i], s->st, hipEventRecordExternal);
-      IeLaunch<R>::spmv(S->dev, s->st);
+      IeLaunch<R>::spmv(dev, s->st);
       if (t && err == hipSuccess) err = hipEventRecordWithFlags(S->ev[4 * i + 1], s->st, hipEventRecordExternal);
-      IeLaunch<R>::alpha(S->dev, s->st);
+      IeLaunch<R>::alpha(dev, s->st);
       if (t && err == hipSuccess) err = hipEventRecordWithFlags(S->ev[4 * i + 2], s->st, hipEventRecordExternal);
-      IeLaunch<R>::update(S->dev, s->st);
+      IeLaunch<R>::update(dev, s->st);
       if (t && err == hipSuccess) err = hipEventRecordWithFlags(S->ev[4 * i + 3], s->st, hipEventRecordExternal);
-      IeLaunch<R>::beta(S->dev, s->st);
+      IeLaunch<R>::beta(dev, s->st);
       if (err == hipSuccess) err = hipGetLastError();
     }
     hipGraph_t gr = nullptr;
@@ -166,68 +167,32 @@ template <typename R> int cg_graph(pbs_solver *s, ImplicitState *S, int timed) {
     if (err == hipSuccess) err = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
     if (gr) (void)hipGraphDestroy(gr);
     if (err != hipSuccess) return set_error(PBS_ERR_HIP, std::string("CG graph capture: ") + hipGetErrorString(err));
-    (t ? S->tgraph : S->graph) = ex;
+    (t ? G.tgraph : G.graph) = ex;
   }
   return PBS_OK;
 }
 
-int check_iparams(const pbs_implicit_params *ip) {
-  if (!ip) return set_error(PBS_ERR_ARG, "null implicit parameters");
-  if (!(ip->cg_tolerance > 0) || !(ip->cg_tolerance < 1)) return set_error(PBS_ERR_ARG, "cg_tolerance must lie in (0, 1)");
-  if (ip->cg_max_iterations < 1) return set_error(PBS_ERR_ARG, "cg_max_iterations must be >= 1");
-  if (!(ip->damping_alpha >= 0) || !(ip->damping_beta >= 0)) return set_error(PBS_ERR_ARG, "damping must be >= 0");
-  return PBS_OK;
-}
-
+// Solves dev's system from the start vector in dev.x (b, dinv already set): the start kernels,
+// then graph replays until the device-side test stops; with profiling on, the SpMV and update
+// launches of the replays are timed.  c receives the final control block.
 template <typename R>
-int implicit_step_impl(pbs_solver *s, const pbs_implicit_params *ip, const float *ext, float *out,
-                       pbs_implicit_stats *stats) {
-  ImplicitState *S = nullptr;
-  int rc = implicit_get(s, &S);
-  if (rc) return rc;
-  DevMesh<R> &m = mesh<R>(s);
-  IeDev &d = S->dev;
-  d.tol2 = ip->cg_tolerance * ip->cg_tolerance;
-  d.maxit = ip->cg_max_iterations;
+int cg_solve(pbs_solver *s, ImplicitState *S, IeDev &dev, CgGraphs &G, double tol, int maxit, IeCtrl &c) {
+  dev.tol2 = tol * tol;
+  dev.maxit = maxit;
   const bool timed = s->prof_period > 0;
-  rc = cg_graph<R>(s, S, timed ? 1 : 0);
+  int rc = cg_graph<R>(s, S, dev, G, timed ? 1 : 0);
   if (rc) return rc;
-  if (ext) {
-    HIPCHK(hipMemcpyAsync(s->stage, ext, sizeof(float) * 3 * (size_t)s->nn, hipMemcpyHostToDevice, s->st));
-    aos_to_planes_f(s->stage, m.ext, m.perm, s->nn, s->st);
-  }
-  IeAsm a{};
-  a.h = s->p.dt;
-  a.alpha = ip->damping_alpha;
-  a.beta = ip->damping_beta;
-  a.raw = 0;
-  a.warm = ip->warm_start ? 1 : 0;
-  a.has_ext = ext ? 1 : 0;
-  HIPCHK(hipEventRecord(s->ev0, s->st));
-  hipEvent_t *ea = timed ? &S->ev[4 * kGraphIters] : nullptr;  // events around the assembly
-  if (ea) HIPCHK(hipEventRecord(ea[0], s->st));
-  IeLaunch<R>::assemble(m, s->td, s->dp, d, a, s->st);
+  IeLaunch<R>::start(dev, s->st);
   HIPCHK(hipGetLastError());
-  if (ea) HIPCHK(hipEventRecord(ea[1], s->st));
-  IeLaunch<R>::start(d, s->st);
-  HIPCHK(hipGetLastError());
-  IeCtrl c{};
-  HIPCHK(hipMemcpyAsync(&c, d.ctrl, sizeof(IeCtrl), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipMemcpyAsync(&c, dev.ctrl, sizeof(IeCtrl), hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
-  if (ea) {
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, ea[0], ea[1]) == hipSuccess) {
-      s->kt_ms[K_IE_ASSEMBLE] += ms;
-      s->kt_n[K_IE_ASSEMBLE] += 1;
-    }
-  }
   long replays = 0;
-  const long max_replays = (ip->cg_max_iterations + kGraphIters - 1) / kGraphIters + 1;
+  const long max_replays = (maxit + kGraphIters - 1) / kGraphIters + 1;
   while (!c.done) {
     if (replays++ >= max_replays) return set_error(PBS_ERR_STATE, "CG loop did not stop within its iteration cap");
     const int k_before = c.k;
-    HIPCHK(hipGraphLaunch(timed ? S->tgraph : S->graph, s->st));
-    HIPCHK(hipMemcpyAsync(&c, d.ctrl, sizeof(IeCtrl), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipGraphLaunch(timed ? G.tgraph : G.graph, s->st));
+    HIPCHK(hipMemcpyAsync(&c, dev.ctrl, sizeof(IeCtrl), hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
     if (timed) {  // iterations that ran: k advanced by one per completed iteration (+1 if stopped in alpha)
       const int ran = std::min(kGraphIters, c.k - k_before + (c.indefinite ? 1 : 0));
@@ -245,16 +210,80 @@ int implicit_step_impl(pbs_solver *s, const pbs_implicit_params *ip, const float
       }
     }
   }
-  if (c.indefinite)
-    return set_error(PBS_ERR_STATE, "backward-Euler step: the system matrix is not positive definite at this state");
-  IeLaunch<R>::finish(m, s->dp, s->p.dt, d, s->st);
+  if (c.indefinite) return set_error(PBS_ERR_STATE, "CG: the system matrix is not positive definite at this state");
+  return PBS_OK;
+}
+
+int check_iparams(const pbs_implicit_params *ip) {
+  if (!ip) return set_error(PBS_ERR_ARG, "null implicit parameters");
+  if (!(ip->cg_tolerance > 0) || !(ip->cg_tolerance < 1)) return set_error(PBS_ERR_ARG, "cg_tolerance must lie in (0, 1)");
+  if (ip->cg_max_iterations < 1) return set_error(PBS_ERR_ARG, "cg_max_iterations must be >= 1");
+  if (!(ip->damping_alpha >= 0) || !(ip->damping_beta >= 0)) return set_error(PBS_ERR_ARG, "damping must be >= 0");
+  return PBS_OK;
+}
+
+int stage_ext(pbs_solver *s, const float *ext) {
+  if (!ext) return PBS_OK;
+  HIPCHK(hipMemcpyAsync(s->stage, ext, sizeof(float) * 3 * (size_t)s->nn, hipMemcpyHostToDevice, s->st));
+  if (s->p.precision == PBS_FP64) aos_to_planes_f(s->stage, s->m64.ext, s->m64.perm, s->nn, s->st);
+  else aos_to_planes_f(s->stage, s->m32.ext, s->m32.perm, s->nn, s->st);
   HIPCHK(hipGetLastError());
+  return PBS_OK;
+}
+
+template <typename R> int finish_step(pbs_solver *s, float *out) {
   HIPCHK(hipEventRecord(s->ev1, s->st));
   if (out) {
-    Launch<R>::planes_to_aos(m.xpos, s->stage, m.perm, s->nn, s->st);
+    Launch<R>::planes_to_aos(mesh<R>(s).xpos, s->stage, mesh<R>(s).perm, s->nn, s->st);
     HIPCHK(hipMemcpyAsync(out, s->stage, sizeof(float) * 3 * (size_t)s->nn, hipMemcpyDeviceToHost, s->st));
   }
   HIPCHK(hipStreamSynchronize(s->st));
+  return PBS_OK;
+}
+
+template <typename R>
+int implicit_step_impl(pbs_solver *s, const pbs_implicit_params *ip, const float *ext, float *out,
+                       pbs_implicit_stats *stats) {
+  ImplicitState *S = nullptr;
+  int rc = implicit_get(s, &S);
+  if (!rc) rc = stage_ext(s, ext);
+  if (rc) return rc;
+  DevMesh<R> &m = mesh<R>(s);
+  IeDev &d = S->dev;
+  IeAsm a{};
+  a.h = s->p.dt;
+  a.alpha = ip->damping_alpha;
+  a.beta = ip->damping_beta;
+  a.raw = 0;
+  a.warm = ip->warm_start ? 1 : 0;
+  a.has_ext = ext ? 1 : 0;
+  const bool timed = s->prof_period > 0;
+  if (timed) {
+    d.tol2 = ip->cg_tolerance * ip->cg_tolerance;
+    d.maxit = ip->cg_max_iterations;
+    rc = cg_graph<R>(s, S, d, S->cg_ie, 1);  // creates the timing events
+    if (rc) return rc;
+  }
+  HIPCHK(hipEventRecord(s->ev0, s->st));
+  hipEvent_t *ea = timed ? &S->ev[4 * kGraphIters] : nullptr;  // events around the assembly
+  if (ea) HIPCHK(hipEventRecord(ea[0], s->st));
+  IeLaunch<R>::assemble(m, s->td, s->dp, d, a, s->st);
+  HIPCHK(hipGetLastError());
+  if (ea) HIPCHK(hipEventRecord(ea[1], s->st));
+  IeCtrl c{};
+  rc = cg_solve<R>(s, S, d, S->cg_ie, ip->cg_tolerance, ip->cg_max_iterations, c);
+  if (rc) return rc;
+  if (ea) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ea[0], ea[1]) == hipSuccess) {
+      s->kt_ms[K_IE_ASSEMBLE] += ms;
+      s->kt_n[K_IE_ASSEMBLE] += 1;
+    }
+  }
+  IeLaunch<R>::finish(m, s->dp, s->p.dt, d, s->st);
+  HIPCHK(hipGetLastError());
+  rc = finish_step<R>(s, out);
+  if (rc) return rc;
   if (stats) {
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
@@ -266,6 +295,109 @@ int implicit_step_impl(pbs_solver *s, const pbs_implicit_params *ip, const float
     stats->rhs_norm = std::sqrt(c.bb);
     stats->gpu_ms = ms;
   }
+  return PBS_OK;
+}
+
+// ---- projective dynamics ----
+template <typename R> int pd_get(pbs_solver *s, ImplicitState **out) {
+  ImplicitState *S = nullptr;
+  int rc = implicit_get(s, &S);
+  if (rc) return rc;
+  if (!S->pd.A) {
+    char *A = nullptr;
+    rc = ialloc(S, &A, sizeof(R) * (size_t)S->slots);
+    if (!rc) rc = ialloc(S, &S->pd_s, 3 * (size_t)s->nn);
+    if (rc) return rc;
+    S->pd = S->dev;  // same layout, vectors and control block
+    S->pd.A = A;
+    S->pd.scalar = 1;
+  }
+  *out = S;
+  return PBS_OK;
+}
+
+int check_pdparams(const pbs_pd_params *pp) {
+  if (!pp) return set_error(PBS_ERR_ARG, "null projective-dynamics parameters");
+  if (!(pp->stiffness > 0)) return set_error(PBS_ERR_ARG, "stiffness must be > 0");
+  if (pp->iterations < 1) return set_error(PBS_ERR_ARG, "iterations must be >= 1");
+  if (!(pp->cg_tolerance > 0) || !(pp->cg_tolerance < 1)) return set_error(PBS_ERR_ARG, "cg_tolerance must lie in (0, 1)");
+  if (pp->cg_max_iterations < 1) return set_error(PBS_ERR_ARG, "cg_max_iterations must be >= 1");
+  return PBS_OK;
+}
+
+template <typename R>
+int pd_step_impl(pbs_solver *s, const pbs_pd_params *pp, const float *ext, float *out, pbs_pd_stats *stats) {
+  ImplicitState *S = nullptr;
+  int rc = pd_get<R>(s, &S);
+  if (!rc) rc = stage_ext(s, ext);
+  if (rc) return rc;
+  DevMesh<R> &m = mesh<R>(s);
+  IeDev &d = S->pd;
+  HIPCHK(hipEventRecord(s->ev0, s->st));
+  IeLaunch<R>::pd_matrix(m, s->dp, pp->stiffness, d, s->st);
+  IeLaunch<R>::pd_begin(m, s->dp, d, S->pd_s, ext ? 1 : 0, s->st);
+  HIPCHK(hipGetLastError());
+  long its = 0;
+  int conv = 1;
+  const bool timed = s->prof_period > 0;
+  if (timed) {
+    d.tol2 = pp->cg_tolerance * pp->cg_tolerance;
+    d.maxit = pp->cg_max_iterations;
+    rc = cg_graph<R>(s, S, d, S->cg_pd, 1);  // creates the timing events
+    if (rc) return rc;
+  }
+  hipEvent_t *ea = timed ? &S->ev[4 * kGraphIters] : nullptr;  // events around the local step
+  for (int it = 0; it < pp->iterations; ++it) {
+    // local step at the current iterate (dev.x), global solve warm-started from it
+    if (ea) HIPCHK(hipEventRecord(ea[0], s->st));
+    IeLaunch<R>::pd_local(m, s->dp, pp->stiffness, d.x, S->pd_s, d.b, s->st);
+    HIPCHK(hipGetLastError());
+    if (ea) HIPCHK(hipEventRecord(ea[1], s->st));
+    IeCtrl c{};
+    rc = cg_solve<R>(s, S, d, S->cg_pd, pp->cg_tolerance, pp->cg_max_iterations, c);
+    if (rc) return rc;
+    if (ea) {  // cg_solve synchronised the stream
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, ea[0], ea[1]) == hipSuccess) {
+        s->kt_ms[K_PD_LOCAL] += ms;
+        s->kt_n[K_PD_LOCAL] += 1;
+      }
+    }
+    its += c.k;
+    conv &= c.converged;
+  }
+  IeLaunch<R>::pd_finish(m, s->dp, pp->damping, d, s->st);
+  HIPCHK(hipGetLastError());
+  rc = finish_step<R>(s, out);
+  if (rc) return rc;
+  if (stats) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
+    std::memset(stats, 0, sizeof(*stats));
+    stats->cg_iterations = (int)its;
+    stats->converged = conv;
+    stats->gpu_ms = ms;
+  }
+  return PBS_OK;
+}
+
+template <typename R> int pd_project_impl(pbs_solver *s, float stiffness, const float *pos, double *proj) {
+  ImplicitState *S = nullptr;
+  int rc = pd_get<R>(s, &S);
+  if (rc) return rc;
+  const size_t n3 = 3 * (size_t)s->nn;
+  // positions (caller order, FP32) -> solver order doubles in the iterate vector
+  std::vector<double> h(n3), o(n3);
+  const std::vector<int> &perm = s->plan.perm;
+  for (int i = 0; i < s->nn; ++i)
+    for (int k = 0; k < 3; ++k) h[3 * (size_t)i + k] = (double)pos[3 * (size_t)perm[i] + k];
+  HIPCHK(hipMemcpyAsync(S->pd.x, h.data(), sizeof(double) * n3, hipMemcpyHostToDevice, s->st));
+  IeLaunch<R>::pd_local(mesh<R>(s), s->dp, stiffness, S->pd.x, nullptr, S->pd.b, s->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(h.data(), S->pd.b, sizeof(double) * n3, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  for (int i = 0; i < s->nn; ++i)
+    for (int k = 0; k < 3; ++k) proj[3 * (size_t)perm[i] + k] = h[3 * (size_t)i + k];
   return PBS_OK;
 }
 
@@ -331,4 +463,31 @@ extern "C" int pbs_implicit_step(pbs_solver *s, const pbs_implicit_params *ip, c
 extern "C" int pbs_stiffness_matrix(pbs_solver *s, int64_t nnz, double *csr_val) {
   if (!s || !csr_val) return set_error(PBS_ERR_ARG, "null argument");
   return s->p.precision == PBS_FP64 ? stiffness_impl<double>(s, nnz, csr_val) : stiffness_impl<float>(s, nnz, csr_val);
+}
+
+extern "C" int pbs_pd_default_params(pbs_pd_params *p) {
+  if (!p) return set_error(PBS_ERR_ARG, "null argument");
+  std::memset(p, 0, sizeof(*p));
+  p->stiffness = 1e6f;  // m_stiffness, m_iterations (ProjDynamic.cpp:4)
+  p->iterations = 5;
+  p->damping = 0.99f;   // ProjDynamic.cpp:477
+  // the unknowns are positions: the fixed nodes' rows (mass 1e8) carry most of |D^-1/2 b|, so the
+  // free nodes need a tighter relative tolerance than the backward-Euler velocity solve
+  p->cg_tolerance = 1e-10;
+  p->cg_max_iterations = 10000;
+  return PBS_OK;
+}
+
+extern "C" int pbs_pd_step(pbs_solver *s, const pbs_pd_params *p, const float *ext, float *out, pbs_pd_stats *st) {
+  if (!s) return set_error(PBS_ERR_ARG, "null handle");
+  int rc = check_pdparams(p);
+  if (rc) return rc;
+  return s->p.precision == PBS_FP64 ? pd_step_impl<double>(s, p, ext, out, st) : pd_step_impl<float>(s, p, ext, out, st);
+}
+
+extern "C" int pbs_pd_project(pbs_solver *s, float stiffness, const float *positions, double *proj) {
+  if (!s || !positions || !proj) return set_error(PBS_ERR_ARG, "null argument");
+  if (!(stiffness > 0)) return set_error(PBS_ERR_ARG, "stiffness must be > 0");
+  return s->p.precision == PBS_FP64 ? pd_project_impl<double>(s, stiffness, positions, proj)
+                                    : pd_project_impl<float>(s, stiffness, positions, proj);
 }

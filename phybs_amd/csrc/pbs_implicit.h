@@ -21,7 +21,7 @@ struct IeCtrl {
 };
 
 struct IeDev {
-  int n, nslice, maxit, pad;
+  int n, nslice, maxit, scalar;    // scalar: one value per slot (block = value x identity)
   double tol2;                     // (relative residual tolerance)^2
   const int *slice_off;            // nslice + 1, in slot groups of 64
   const int *row_len;              // N
@@ -51,18 +51,30 @@ template <typename R> struct IeLaunch {
   static void update(const IeDev &d, hipStream_t st);
   static void beta(const IeDev &d, hipStream_t st);
   static void finish(const DevMesh<R> &m, const DevParams &p, float h, const IeDev &d, hipStream_t st);
+  // projective dynamics: global matrix, momentum term + iterate start, local step, velocity update
+  static void pd_matrix(const DevMesh<R> &m, const DevParams &p, float stiffness, const IeDev &d, hipStream_t st);
+  static void pd_begin(const DevMesh<R> &m, const DevParams &p, const IeDev &d, double *srhs, int has_ext,
+                       hipStream_t st);
+  static void pd_local(const DevMesh<R> &m, const DevParams &p, float stiffness, const double *pos, const double *srhs,
+                       double *out, hipStream_t st);
+  static void pd_finish(const DevMesh<R> &m, const DevParams &p, float damping, const IeDev &d, hipStream_t st);
+};
+
+// Captured CG iterations for one matrix (plain replay and the timed variant).
+struct CgGraphs {
+  hipGraphExec_t graph = nullptr, tgraph = nullptr;
+  double key[2] = {0, 0};  // tol2, maxit the graphs were built with
 };
 
 struct ImplicitState {
   int nslice = 0;
   long slots = 0;
-  IeDev dev{};
+  IeDev dev{};                                      // backward Euler: 3x3 blocks
+  IeDev pd{};                                       // projective dynamics: same layout and vectors, scalar values
+  double *pd_s = nullptr;                           // projective dynamics: momentum term (3N)
   std::vector<int> slice_off_h, row_len_h, bcol_h;  // host copies (stiffness export)
-  hipGraphExec_t graph = nullptr;
-  int graph_iters = 0;
-  double graph_key[2] = {0, 0};                    // tol2, maxit the graph was built with
-  hipGraphExec_t tgraph = nullptr;                 // timed variant: events around each SpMV
-  std::vector<hipEvent_t> ev;
+  CgGraphs cg_ie, cg_pd;
+  std::vector<hipEvent_t> ev;                       // timing events of the timed replays
   std::vector<void *> allocs;
 };
 

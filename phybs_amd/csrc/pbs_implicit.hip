@@ -49,6 +49,30 @@ __device__ __forceinline__ double partial_sum(const double *part, int nb, int c,
   return t;
 }
 
+// y += (row n of A) v.  BLK = 3: 3x3 blocks (9 values per slot); BLK = 1: one value per slot,
+// the block being that value times the 3x3 identity.  get(col, v0, v1, v2) supplies v at a column.
+template <typename R, int BLK, typename Get>
+__device__ __forceinline__ void row_apply(const IeDev &d, int n, Get get, double *y) {
+  const R *__restrict__ A = (const R *)d.A;
+  const int lane = n & 63, g0 = d.slice_off[n >> 6], len = d.row_len[n];
+  for (int j = 0; j < len; ++j) {
+    const int col = d.bcol[(size_t)(g0 + j) * 64 + lane];
+    double x0, x1, x2;
+    get(col, x0, x1, x2);
+    if (BLK == 3) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        y[q] += (double)A[val_idx(g0 + j, 3 * q, lane)] * x0 + (double)A[val_idx(g0 + j, 3 * q + 1, lane)] * x1 +
+                (double)A[val_idx(g0 + j, 3 * q + 2, lane)] * x2;
+    } else {
+      const double a = (double)A[(size_t)(g0 + j) * 64 + lane];
+      y[0] += a * x0;
+      y[1] += a * x1;
+      y[2] += a * x2;
+    }
+  }
+}
+
 // Slot (within the row) of column `col`: the row's columns are ascending.
 __device__ __forceinline__ int find_col(const int *__restrict__ bcol, int g0, int lane, int len, int col) {
   int lo = 0, hi = len - 1;
@@ -237,22 +261,18 @@ __global__ void __launch_bounds__(kBlock)
 // r = b - A x0, z = D^-1 r, p = z; partials r.z, r.r, b.D^-1.b.  The stopping test uses the
 // Jacobi-scaled norms |D^-1/2 r| = sqrt(r.z) and |D^-1/2 b|: fixed nodes carry a 1e8 mass and
 // spring, and in the plain 2-norm their rows would swamp the free nodes' residual.
-template <typename R> __global__ void __launch_bounds__(kBlock) k_ie_start(IeDev d) {
+template <typename R, int BLK> __global__ void __launch_bounds__(kBlock) k_ie_start(IeDev d) {
   __shared__ double lds[4];
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   double rz = 0.0, rr = 0.0, bb = 0.0;
   if (n < d.n) {
-    const R *__restrict__ A = (const R *)d.A;
-    const int lane = n & 63, g0 = d.slice_off[n >> 6], len = d.row_len[n];
     double y[3] = {0.0, 0.0, 0.0};
-    for (int j = 0; j < len; ++j) {
-      const int col = d.bcol[(size_t)(g0 + j) * 64 + lane];
-      const double x0 = d.x[3 * (size_t)col], x1 = d.x[3 * (size_t)col + 1], x2 = d.x[3 * (size_t)col + 2];
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        y[k] += (double)A[val_idx(g0 + j, 3 * k, lane)] * x0 + (double)A[val_idx(g0 + j, 3 * k + 1, lane)] * x1 +
-                (double)A[val_idx(g0 + j, 3 * k + 2, lane)] * x2;
-    }
+    const double *__restrict__ xv = d.x;
+    row_apply<R, BLK>(d, n, [&](int col, double &x0, double &x1, double &x2) {
+      x0 = xv[3 * (size_t)col];
+      x1 = xv[3 * (size_t)col + 1];
+      x2 = xv[3 * (size_t)col + 2];
+    }, y);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const size_t i = 3 * (size_t)n + k;
@@ -297,7 +317,7 @@ __global__ void __launch_bounds__(1024) k_ie_start_ctrl(IeDev d, int nb) {
 
 // Iteration k: p_k = z + beta p_{k-1} (formed on the fly for every column, stored for the own
 // row), Ap = A p_k, partial p_k . Ap.
-template <typename R> __global__ void __launch_bounds__(kBlock) k_ie_spmv(IeDev d) {
+template <typename R, int BLK> __global__ void __launch_bounds__(kBlock) k_ie_spmv(IeDev d) {
   __shared__ double lds[4];
   const IeCtrl *c = d.ctrl;
   if (c->done) return;
@@ -308,26 +328,19 @@ template <typename R> __global__ void __launch_bounds__(kBlock) k_ie_spmv(IeDev 
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   double pap = 0.0;
   if (n < d.n) {
-    const R *__restrict__ A = (const R *)d.A;
-    const int lane = n & 63, g0 = d.slice_off[n >> 6], len = d.row_len[n];
     double y[3] = {0.0, 0.0, 0.0};
-    for (int j = 0; j < len; ++j) {
-      const int col = d.bcol[(size_t)(g0 + j) * 64 + lane];
-      double x0, x1, x2;
+    const double *__restrict__ z = d.z;
+    row_apply<R, BLK>(d, n, [&](int col, double &x0, double &x1, double &x2) {
       if (k == 0) {
         x0 = pout[3 * (size_t)col];
         x1 = pout[3 * (size_t)col + 1];
         x2 = pout[3 * (size_t)col + 2];
       } else {
-        x0 = d.z[3 * (size_t)col] + beta * pin[3 * (size_t)col];
-        x1 = d.z[3 * (size_t)col + 1] + beta * pin[3 * (size_t)col + 1];
-        x2 = d.z[3 * (size_t)col + 2] + beta * pin[3 * (size_t)col + 2];
+        x0 = z[3 * (size_t)col] + beta * pin[3 * (size_t)col];
+        x1 = z[3 * (size_t)col + 1] + beta * pin[3 * (size_t)col + 1];
+        x2 = z[3 * (size_t)col + 2] + beta * pin[3 * (size_t)col + 2];
       }
-#pragma unroll
-      for (int q = 0; q < 3; ++q)
-        y[q] += (double)A[val_idx(g0 + j, 3 * q, lane)] * x0 + (double)A[val_idx(g0 + j, 3 * q + 1, lane)] * x1 +
-                (double)A[val_idx(g0 + j, 3 * q + 2, lane)] * x2;
-    }
+    }, y);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       const size_t i = 3 * (size_t)n + q;
@@ -419,6 +432,151 @@ __global__ void k_ie_finish(DevParams p, float h, IeDev d, R *__restrict__ xpos,
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Projective dynamics (ProjDynamic.cpp).  The global matrix (buildGlobalSolverMatrix, :46-96) is
+// G = M / t^2 + sum_k w_k Ac_k^T Ac_k with w_k = stiffness * vol_k; Ac_k^T Ac_k couples corners i, j
+// of tet k by (c_i . c_j) times the 3x3 identity, so G is stored with one value per slot.
+// ---------------------------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_pd_matrix(DevParams p, float stiffness, IeDev d, const int4 *__restrict__ tets, const float *__restrict__ dmi,
+                const float *__restrict__ vol, const int *__restrict__ inc_off, const int *__restrict__ inc,
+                const float *__restrict__ mass) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= d.n) return;
+  R *__restrict__ A = (R *)d.A;
+  const int T = p.nt;
+  const int lane = n & 63;
+  const int g0 = d.slice_off[n >> 6];
+  const int width = d.slice_off[(n >> 6) + 1] - g0;
+  const int len = d.row_len[n];
+  for (int j = 0; j < width; ++j) A[(size_t)(g0 + j) * 64 + lane] = (R)0;
+  for (int e = inc_off[n]; e < inc_off[n + 1]; ++e) {
+    const int t = inc[e] >> 2, ci = inc[e] & 3;
+    const int4 q4 = tets[t];
+    const int qn[4] = {q4.x, q4.y, q4.z, q4.w};
+    R Dmi[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Dmi[k] = (R)dmi[(size_t)k * T + t];
+    const R w = (R)stiffness * (R)vol[t];
+    R c[4][3];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) shape_vec(Dmi, m, c[m]);
+    R cs[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) cs[k] = ci == 0 ? c[0][k] : ci == 1 ? c[1][k] : ci == 2 ? c[2][k] : c[3][k];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int slot_g = g0 + find_col(d.bcol, g0, lane, len, qn[m]);
+      A[(size_t)slot_g * 64 + lane] += w * (cs[0] * c[m][0] + cs[1] * c[m][1] + cs[2] * c[m][2]);
+    }
+  }
+  const R h = (R)p.h;
+  const R t2inv = (R)1 / (h * h);
+  const int ds = d.diag_slot[n];
+  A[ds] += (R)mass[n] * t2inv;
+  const double di = 1.0 / (double)A[ds];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) d.dinv[3 * (size_t)n + k] = di;
+}
+
+// Momentum term of doProjDynamics (ProjDynamic.cpp:442-460): s = (x + t v + t^2 m^-1 f_ext) m / t^2
+// with f_ext = gravity + external forces (TetMesh::computeExternalForces; fixed nodes have
+// m^-1 = 0); the iterate starts at x.
+template <typename R>
+__global__ void k_pd_begin(DevParams p, IeDev d, double *__restrict__ srhs, const R *__restrict__ xpos,
+                           const R *__restrict__ vel, const float *__restrict__ mass, const float *__restrict__ gforce,
+                           const float *__restrict__ ext, int has_ext, const int *__restrict__ mask) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= d.n) return;
+  const R t = (R)p.h, m = (R)mass[n];
+  const R im = mask[n] ? (R)0 : (R)1 / m;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const size_t i = (size_t)k * p.nn + n;
+    R f = (R)gforce[i];
+    if (has_ext) f += (R)ext[i];
+    const R y = xpos[i] + t * vel[i] + t * t * im * f;
+    srhs[3 * (size_t)n + k] = (double)(y * m / (t * t));
+    d.x[3 * (size_t)n + k] = (double)xpos[i];
+  }
+}
+
+// The projection rotation of one tet: R = U V^T of F's SVD with the signs the reference fixes
+// (ProjDynamic.cpp:359-368: a column of U or V whose determinant is negative is negated, column 0
+// of the singular values sorted in decreasing order).  With svd3's rotations U, V and the sign of
+// det F on the last singular value this is U V^T for det F >= 0 and U diag(-1, 1, -1) V^T otherwise.
+template <typename R> __device__ __forceinline__ void pd_rotation(const R *F, R *Rot) {
+  R U[9], S[3], V[9];
+  svd3(F, U, S, V);
+  const R s0 = S[2] < (R)0 ? (R)-1 : (R)1;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Rot[3 * i + j] = s0 * U[3 * i] * V[3 * j] + U[3 * i + 1] * V[3 * j + 1] + s0 * U[3 * i + 2] * V[3 * j + 2];
+}
+
+// Local step (projectLocalConstraints2, ProjDynamic.cpp:223-308): per node, over its tets in
+// ascending order, the projection w_k Ac_k^T vec(R_k) restricted to its corner, i.e.
+// w_k R_k c_i; out = s + that sum (s may be null: the projection alone).  Positions are read
+// from pos (3N node-major doubles).
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_pd_local(DevParams p, float stiffness, int n_nodes, const double *__restrict__ pos, const double *__restrict__ srhs,
+               double *__restrict__ out, const int4 *__restrict__ tets, const float *__restrict__ dmi,
+               const float *__restrict__ vol, const int *__restrict__ inc_off, const int *__restrict__ inc) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n_nodes) return;
+  const int T = p.nt;
+  R acc[3] = {(R)0, (R)0, (R)0};
+  for (int e = inc_off[n]; e < inc_off[n + 1]; ++e) {
+    const int t = inc[e] >> 2, ci = inc[e] & 3;
+    const int4 q4 = tets[t];
+    const int qn[4] = {q4.x, q4.y, q4.z, q4.w};
+    R X[4][3];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) X[c][k] = (R)pos[3 * (size_t)qn[c] + k];
+    R Dmi[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Dmi[k] = (R)dmi[(size_t)k * T + t];
+    R Ds[9], F[9], Rot[9];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      Ds[3 * k + 0] = X[1][k] - X[0][k];
+      Ds[3 * k + 1] = X[2][k] - X[0][k];
+      Ds[3 * k + 2] = X[3][k] - X[0][k];
+    }
+    mul3(Ds, Dmi, F);
+    pd_rotation(F, Rot);
+    R c[3];
+    shape_vec(Dmi, ci, c);
+    const R w = (R)stiffness * (R)vol[t];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) acc[k] += w * (Rot[3 * k] * c[0] + Rot[3 * k + 1] * c[1] + Rot[3 * k + 2] * c[2]);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const size_t i = 3 * (size_t)n + k;
+    out[i] = srhs ? (double)((R)srhs[i] + acc[k]) : (double)acc[k];
+  }
+}
+
+// v = damping (x_new - x) / t, x = x_new (ProjDynamic.cpp:477-478), every node.
+template <typename R>
+__global__ void k_pd_finish(DevParams p, float damping, IeDev d, R *__restrict__ xpos, R *__restrict__ vel) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= d.n) return;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const size_t i = (size_t)k * p.nn + n;
+    const R xn = (R)d.x[3 * (size_t)n + k];
+    vel[i] = (xn - xpos[i]) / (R)p.h * (R)damping;
+    xpos[i] = xn;
+  }
+}
+
 inline int nb_of(int n) { return (n + kBlock - 1) / kBlock; }
 
 }  // namespace
@@ -435,12 +593,14 @@ void IeLaunch<R>::assemble(const DevMesh<R> &m, const TileDev &td, const DevPara
 }
 
 template <typename R> void IeLaunch<R>::start(const IeDev &d, hipStream_t st) {
-  hipLaunchKernelGGL(k_ie_start<R>, dim3(nb_of(d.n)), dim3(kBlock), 0, st, d);
+  if (d.scalar) hipLaunchKernelGGL((k_ie_start<R, 1>), dim3(nb_of(d.n)), dim3(kBlock), 0, st, d);
+  else hipLaunchKernelGGL((k_ie_start<R, 3>), dim3(nb_of(d.n)), dim3(kBlock), 0, st, d);
   hipLaunchKernelGGL(k_ie_start_ctrl, dim3(1), dim3(1024), 0, st, d, nb_of(d.n));
 }
 
 template <typename R> void IeLaunch<R>::spmv(const IeDev &d, hipStream_t st) {
-  hipLaunchKernelGGL(k_ie_spmv<R>, dim3(nb_of(d.n)), dim3(kBlock), 0, st, d);
+  if (d.scalar) hipLaunchKernelGGL((k_ie_spmv<R, 1>), dim3(nb_of(d.n)), dim3(kBlock), 0, st, d);
+  else hipLaunchKernelGGL((k_ie_spmv<R, 3>), dim3(nb_of(d.n)), dim3(kBlock), 0, st, d);
 }
 template <typename R> void IeLaunch<R>::alpha(const IeDev &d, hipStream_t st) {
   hipLaunchKernelGGL(k_ie_alpha, dim3(1), dim3(1024), 0, st, d, nb_of(d.n));
@@ -455,6 +615,28 @@ template <typename R> void IeLaunch<R>::beta(const IeDev &d, hipStream_t st) {
 template <typename R>
 void IeLaunch<R>::finish(const DevMesh<R> &m, const DevParams &p, float h, const IeDev &d, hipStream_t st) {
   hipLaunchKernelGGL(k_ie_finish<R>, dim3(nb_of(d.n)), dim3(kBlock), 0, st, p, h, d, m.xpos, m.v);
+}
+
+template <typename R>
+void IeLaunch<R>::pd_matrix(const DevMesh<R> &m, const DevParams &p, float stiffness, const IeDev &d, hipStream_t st) {
+  hipLaunchKernelGGL(k_pd_matrix<R>, dim3(nb_of(d.n)), dim3(kBlock), 0, st, p, stiffness, d, m.tets, m.dmi, m.vol,
+                     m.inc_off, m.inc, m.mass);
+}
+template <typename R>
+void IeLaunch<R>::pd_begin(const DevMesh<R> &m, const DevParams &p, const IeDev &d, double *srhs, int has_ext,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(k_pd_begin<R>, dim3(nb_of(d.n)), dim3(kBlock), 0, st, p, d, srhs, m.xpos, m.v, m.mass, m.gforce,
+                     m.ext, has_ext, m.mask);
+}
+template <typename R>
+void IeLaunch<R>::pd_local(const DevMesh<R> &m, const DevParams &p, float stiffness, const double *pos,
+                           const double *srhs, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_pd_local<R>, dim3(nb_of(p.nn)), dim3(kBlock), 0, st, p, stiffness, p.nn, pos, srhs, out, m.tets,
+                     m.dmi, m.vol, m.inc_off, m.inc);
+}
+template <typename R>
+void IeLaunch<R>::pd_finish(const DevMesh<R> &m, const DevParams &p, float damping, const IeDev &d, hipStream_t st) {
+  hipLaunchKernelGGL(k_pd_finish<R>, dim3(nb_of(d.n)), dim3(kBlock), 0, st, p, damping, d, m.xpos, m.v);
 }
 
 template struct IeLaunch<float>;

@@ -316,6 +316,95 @@ def test_implicit_system_matrix_and_rhs():
 
 
 # ---------------------------------------------------------------------------------------------
+# projective dynamics (local step, global matrix)
+def test_pd_rotation_polar_and_inverted():
+    from scipy.linalg import polar
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        F = np.eye(3) + 0.4 * rng.standard_normal((3, 3))
+        R = O.pd_rotation(F)
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-12) and np.linalg.det(R) == pytest.approx(1.0, abs=1e-12)
+        if np.linalg.det(F) > 0:
+            assert np.allclose(R, polar(F)[0], atol=1e-10)
+        else:  # numpy's SVD with the reference's sign fixes (columns 0 of U / V)
+            U, S, Vh = np.linalg.svd(F)
+            V = Vh.T.copy()
+            if np.linalg.det(U) < 0:
+                U[:, 0] *= -1
+            if np.linalg.det(V) < 0:
+                V[:, 0] *= -1
+            assert np.allclose(R, U @ V.T, atol=1e-10)
+
+
+def pd_case():
+    rest, tets = M.bar(1, 1, 2, 0.05)
+    mask = M.clamp_x0(rest)
+    pre = O.precompute(rest, tets, mask)
+    rng = np.random.default_rng(4)
+    x = rest.astype(np.float64) + 0.01 * rng.standard_normal(rest.shape)
+    x[tets[0][0]] = 2 * x[tets[0][1:]].mean(axis=0) - x[tets[0][0]]  # invert one tet
+    return rest, tets, mask, pre, x
+
+
+def def_grads(x, tets, pre):
+    D = pre["dminv"].astype(np.float64).reshape(-1, 3, 3)
+    Ds = np.stack([x[tets[:, 1]] - x[tets[:, 0]], x[tets[:, 2]] - x[tets[:, 0]], x[tets[:, 3]] - x[tets[:, 0]]], axis=2)
+    return Ds @ D
+
+
+def test_pd_project_fd():
+    """The local step is the gradient of sum_k w_k <F_k(x), R_k> with the rotations frozen."""
+    rest, tets, mask, pre, x = pd_case()
+    k = 1e6
+    F0 = def_grads(x, tets, pre)
+    assert (np.linalg.det(F0) < 0).any() and (np.linalg.det(F0) > 0).any()
+    Rs = np.stack([O.pd_rotation(F) for F in F0])
+    w = k * pre["vol"].astype(np.float64)
+
+    def phi(y):
+        return float(np.sum(w * np.einsum("kab,kab->k", def_grads(y, tets, pre), Rs)))
+
+    got = O.pd_project(x, tets, pre, k)
+    fd = np.zeros_like(x)
+    for i in range(len(x)):
+        for d in range(3):
+            a, b = x.copy(), x.copy()
+            a[i, d] += 1e-4
+            b[i, d] -= 1e-4
+            fd[i, d] = (phi(a) - phi(b)) / 2e-4
+    assert np.allclose(got, fd, rtol=1e-7, atol=1e-7 * np.abs(fd).max())
+    s = np.arange(x.size, dtype=np.float64).reshape(x.shape)
+    assert np.allclose(O.pd_project(x, tets, pre, k, s=s), got + s, rtol=1e-14)
+
+
+def test_pd_matrix_fd():
+    """G - M/t^2 is the Hessian of sum_k w_k |F_k(x)|^2 / 2 (a quadratic: second differences are exact)."""
+    rest, tets, mask, pre, x = pd_case()
+    k, t = 1e6, float(np.float32(0.03))
+    rowptr, col, g = O.pd_matrix(len(rest), tets, pre, k, t)
+    G = dense(rowptr, col, g, 3 * len(rest))
+    w = k * pre["vol"].astype(np.float64)
+
+    def q(y):
+        return float(np.sum(w * np.einsum("kab,kab->k", def_grads(y.reshape(-1, 3), tets, pre),
+                                          def_grads(y.reshape(-1, 3), tets, pre))) / 2)
+
+    n3 = 3 * len(rest)
+    y0 = x.reshape(-1)
+    H = np.zeros((n3, n3))
+    for i in range(n3):
+        for j in range(n3):
+            e = np.zeros((4, n3))
+            e[0, i] += 1; e[0, j] += 1
+            e[1, i] += 1; e[1, j] -= 1
+            e[2, i] -= 1; e[2, j] += 1
+            e[3, i] -= 1; e[3, j] -= 1
+            H[i, j] = (q(y0 + e[0]) - q(y0 + e[1]) - q(y0 + e[2]) + q(y0 + e[3])) / 4
+    m3 = np.repeat(pre["mass"].astype(np.float64), 3)
+    assert np.allclose(G - np.diag(m3 / (t * t)), H, rtol=1e-9, atol=1e-9 * np.abs(H).max())
+
+
+# ---------------------------------------------------------------------------------------------
 # descent loop
 def test_step_mass_only_closed_form():
     """With E = 0 the incremental potential is 1/2 m |x - x~|^2 / h^2 - m g.x: minimiser
