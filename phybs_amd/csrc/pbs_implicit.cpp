@@ -94,7 +94,7 @@ int implicit_get(pbs_solver *s, ImplicitState **out) {
   }
   const size_t rsz = s->p.precision == PBS_FP64 ? 8 : 4;
   const size_t n3 = 3 * (size_t)N;
-  const int nb = (N + 255) / 256;
+  const int nb = (3 * N + 255) / 256;  // partial sums: up to one per 256 vector entries
   IeDev &d = S->dev;
   d.n = N;
   d.nslice = S->nslice;

@@ -29,7 +29,7 @@ struct IeDev {
   const int *diag_slot;            // N: slot index of each row's diagonal block
   void *A;                         // 9 x slots values of the solver's real type
   double *b, *x, *r, *z, *p0, *p1, *Ap, *dinv;  // 3N, node-major (x, y, z of a node together)
-  double *part;                    // 4 partial sums per 256-row block
+  double *part;                    // 4 partial sums per 256-row block (per 256 entries in updates)
   IeCtrl *ctrl;
 };
 

@@ -20,6 +20,17 @@ __device__ __forceinline__ size_t val_idx(long group, int q, int lane) {
   return ((size_t)group * 9 + q) * 64 + lane;
 }
 
+// XCD-contiguous block order: workgroup i runs on XCD i % 8; giving each XCD a contiguous range
+// of row blocks keeps a row's neighbours (mostly in the same or the adjacent blocks, in Morton
+// order) in that XCD's L2.
+__device__ __forceinline__ int xcd_block(int bid, int nblk) {
+  const int per = (nblk + 7) >> 3;
+  const int xcd = bid & 7, k = bid >> 3;
+  const int full = nblk - 8 * (per - 1);
+  const int start = xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1);
+  return start + k;
+}
+
 // Sum of v over the 256 threads of the block in a fixed order (wave tree, then the 4 waves).
 __device__ __forceinline__ double block_sum256(double v, double *lds) {
 #pragma unroll
@@ -54,8 +65,11 @@ __device__ __forceinline__ double partial_sum(const double *part, int nb, int c,
 template <typename R, int BLK, typename Get>
 __device__ __forceinline__ void row_apply(const IeDev &d, int n, Get get, double *y) {
   const R *__restrict__ A = (const R *)d.A;
-  const int lane = n & 63, g0 = d.slice_off[n >> 6], len = d.row_len[n];
-  for (int j = 0; j < len; ++j) {
+  // all lanes run to the slice width (padding slots hold zeros and the row's own column), so the
+  // loop is uniform across the wavefront and can be unrolled to overlap its loads
+  const int lane = n & 63, g0 = d.slice_off[n >> 6], width = d.slice_off[(n >> 6) + 1] - g0;
+#pragma unroll 2
+  for (int j = 0; j < width; ++j) {
     const int col = d.bcol[(size_t)(g0 + j) * 64 + lane];
     double x0, x1, x2;
     get(col, x0, x1, x2);
@@ -263,7 +277,8 @@ __global__ void __launch_bounds__(kBlock)
 // spring, and in the plain 2-norm their rows would swamp the free nodes' residual.
 template <typename R, int BLK> __global__ void __launch_bounds__(kBlock) k_ie_start(IeDev d) {
   __shared__ double lds[4];
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int n = blk * kBlock + threadIdx.x;
   double rz = 0.0, rr = 0.0, bb = 0.0;
   if (n < d.n) {
     double y[3] = {0.0, 0.0, 0.0};
@@ -291,9 +306,9 @@ template <typename R, int BLK> __global__ void __launch_bounds__(kBlock) k_ie_st
   rr = block_sum256(rr, lds);
   bb = block_sum256(bb, lds);
   if (threadIdx.x == 0) {
-    d.part[4 * (size_t)blockIdx.x + 0] = rz;
-    d.part[4 * (size_t)blockIdx.x + 1] = rr;
-    d.part[4 * (size_t)blockIdx.x + 2] = bb;
+    d.part[4 * (size_t)blk + 0] = rz;
+    d.part[4 * (size_t)blk + 1] = rr;
+    d.part[4 * (size_t)blk + 2] = bb;
   }
 }
 
@@ -325,7 +340,8 @@ template <typename R, int BLK> __global__ void __launch_bounds__(kBlock) k_ie_sp
   const double beta = c->beta;
   const double *__restrict__ pin = (k & 1) ? d.p0 : d.p1;  // p_{k-1} (k >= 1)
   double *__restrict__ pout = (k & 1) ? d.p1 : d.p0;       // p_k
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int n = blk * kBlock + threadIdx.x;
   double pap = 0.0;
   if (n < d.n) {
     double y[3] = {0.0, 0.0, 0.0};
@@ -351,7 +367,7 @@ template <typename R, int BLK> __global__ void __launch_bounds__(kBlock) k_ie_sp
     }
   }
   pap = block_sum256(pap, lds);
-  if (threadIdx.x == 0) d.part[4 * (size_t)blockIdx.x + 3] = pap;
+  if (threadIdx.x == 0) d.part[4 * (size_t)blk + 3] = pap;
 }
 
 __global__ void __launch_bounds__(1024) k_ie_alpha(IeDev d, int nb) {
@@ -371,27 +387,24 @@ __global__ void __launch_bounds__(1024) k_ie_alpha(IeDev d, int nb) {
   }
 }
 
-// x += alpha p_k, r -= alpha Ap, z = D^-1 r; partials r.z, r.r
+// x += alpha p_k, r -= alpha Ap, z = D^-1 r; partials r.z, r.r.  One thread per vector entry
+// (3N of them), so that every load and store of a wavefront is one contiguous 512-byte access.
 __global__ void __launch_bounds__(kBlock) k_ie_update(IeDev d) {
   __shared__ double lds[4];
   const IeCtrl *c = d.ctrl;
   if (c->done) return;
   const double alpha = c->alpha;
   const double *__restrict__ pk = (c->k & 1) ? d.p1 : d.p0;
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   double rz = 0.0, rr = 0.0;
-  if (n < d.n) {
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const size_t i = 3 * (size_t)n + q;
-      d.x[i] += alpha * pk[i];
-      const double r = d.r[i] - alpha * d.Ap[i];
-      const double z = d.dinv[i] * r;
-      d.r[i] = r;
-      d.z[i] = z;
-      rz += r * z;
-      rr += r * r;
-    }
+  if (i < 3 * (size_t)d.n) {
+    d.x[i] += alpha * pk[i];
+    const double r = d.r[i] - alpha * d.Ap[i];
+    const double z = d.dinv[i] * r;
+    d.r[i] = r;
+    d.z[i] = z;
+    rz = r * z;
+    rr = r * r;
   }
   rz = block_sum256(rz, lds);
   rr = block_sum256(rr, lds);
@@ -606,10 +619,10 @@ template <typename R> void IeLaunch<R>::alpha(const IeDev &d, hipStream_t st) {
   hipLaunchKernelGGL(k_ie_alpha, dim3(1), dim3(1024), 0, st, d, nb_of(d.n));
 }
 template <typename R> void IeLaunch<R>::update(const IeDev &d, hipStream_t st) {
-  hipLaunchKernelGGL(k_ie_update, dim3(nb_of(d.n)), dim3(kBlock), 0, st, d);
+  hipLaunchKernelGGL(k_ie_update, dim3(nb_of(3 * d.n)), dim3(kBlock), 0, st, d);
 }
 template <typename R> void IeLaunch<R>::beta(const IeDev &d, hipStream_t st) {
-  hipLaunchKernelGGL(k_ie_beta, dim3(1), dim3(1024), 0, st, d, nb_of(d.n));
+  hipLaunchKernelGGL(k_ie_beta, dim3(1), dim3(1024), 0, st, d, nb_of(3 * d.n));
 }
 
 template <typename R>
