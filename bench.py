@@ -197,7 +197,7 @@ def implicit_report(P, rest, tets, mask, material, grav, steps=3):
     """Backward-Euler steps (the reference's GPU implicit path) on the benchmark bar in FP32 from
     rest under gravity: steps/s, CG iterations, and the SpMV kernel against HBM bandwidth.  SpMV
     algorithmic bytes per launch: every stored 3x3 block once (9 FP32 values + a 4-byte column
-    index) plus 4 vectors of 3 doubles per node (z and p_{k-1} read, p_k and A p_k written)."""
+    index) plus 2 vectors of 3 doubles per node (p read, A p written)."""
     prm = P.default_params(dt=1.0 / 30.0, grav=grav, material=material)
     s = P.Solver(rest, tets, mask, prm)
     ip = P.implicit_params()
@@ -212,7 +212,7 @@ def implicit_report(P, rest, tets, mask, material, grav, steps=3):
     t32 = np.ascontiguousarray(tets, np.int32)
     nblocks = P.lib().pbs_csr_pattern(len(rest), len(t32), t32.ctypes.data, None, None, None, None) // 9
     nn = len(rest)
-    spmv_bytes = nblocks * (9 * 4 + 4) + 4 * 24 * nn
+    spmv_bytes = nblocks * (9 * 4 + 4) + 2 * 24 * nn
     upd_bytes = 8 * 24 * nn
     spmv_ms, upd_ms = kt["k_ie_spmv"][0], kt["k_ie_update"][0]
     ach = spmv_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else 0.0
@@ -249,7 +249,7 @@ def pd_report(P, rest, tets, mask, grav, steps=2):
     loc_ms, spmv_ms = kt["k_pd_local"][0], kt["k_ie_spmv"][0]
     t32 = np.ascontiguousarray(tets, np.int32)
     nblocks = P.lib().pbs_csr_pattern(nn, len(t32), t32.ctypes.data, None, None, None, None) // 9
-    spmv_bytes = nblocks * 8 + 4 * 24 * nn
+    spmv_bytes = nblocks * 8 + 2 * 24 * nn  # one FP32 value + column per block, p read, A p written
     return {
         "steps": steps, "steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps, "dtype": "f32 matrix, f64 CG",
         "rounds_per_step": pp.iterations, "cg_tolerance": pp.cg_tolerance,

@@ -105,8 +105,8 @@ int implicit_get(pbs_solver *s, ImplicitState **out) {
   if (!rc) rc = ialloc(S, &bc, S->slots);
   if (!rc) rc = ialloc(S, &dsl, N);
   if (!rc) rc = ialloc(S, &A, 9 * rsz * S->slots);
-  double **vec[8] = {&d.b, &d.x, &d.r, &d.z, &d.p0, &d.p1, &d.Ap, &d.dinv};
-  for (int i = 0; i < 8 && !rc; ++i) rc = ialloc(S, vec[i], n3);
+  double **vec[7] = {&d.b, &d.x, &d.r, &d.z, &d.p, &d.Ap, &d.dinv};
+  for (int i = 0; i < 7 && !rc; ++i) rc = ialloc(S, vec[i], n3);
   if (!rc) rc = ialloc(S, &d.part, 4 * (size_t)nb);
   if (!rc) rc = ialloc(S, &d.ctrl, 1);
   if (rc) {

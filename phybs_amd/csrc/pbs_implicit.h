@@ -28,7 +28,7 @@ struct IeDev {
   const int *bcol;                 // slots
   const int *diag_slot;            // N: slot index of each row's diagonal block
   void *A;                         // 9 x slots values of the solver's real type
-  double *b, *x, *r, *z, *p0, *p1, *Ap, *dinv;  // 3N, node-major (x, y, z of a node together)
+  double *b, *x, *r, *z, *p, *Ap, *dinv;  // 3N, node-major (x, y, z of a node together)
   double *part;                    // 4 partial sums per 256-row block (per 256 entries in updates)
   IeCtrl *ctrl;
 };
@@ -45,7 +45,7 @@ template <typename R> struct IeLaunch {
   static void assemble(const DevMesh<R> &m, const TileDev &td, const DevParams &p, const IeDev &d, const IeAsm &a,
                        hipStream_t st);
   static void start(const IeDev &d, hipStream_t st);
-  // one CG iteration = spmv, alpha, update, beta (each a kernel)
+  // one CG iteration = spmv, alpha, update, beta (+ the search-direction update), each a kernel
   static void spmv(const IeDev &d, hipStream_t st);
   static void alpha(const IeDev &d, hipStream_t st);
   static void update(const IeDev &d, hipStream_t st);

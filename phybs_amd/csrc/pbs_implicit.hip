@@ -296,7 +296,7 @@ template <typename R, int BLK> __global__ void __launch_bounds__(kBlock) k_ie_st
       const double z = d.dinv[i] * r;
       d.r[i] = r;
       d.z[i] = z;
-      d.p0[i] = z;
+      d.p[i] = z;
       rz += r * z;
       rr += r * r;
       bb += b * b * d.dinv[i];
@@ -330,40 +330,27 @@ __global__ void __launch_bounds__(1024) k_ie_start_ctrl(IeDev d, int nb) {
   }
 }
 
-// Iteration k: p_k = z + beta p_{k-1} (formed on the fly for every column, stored for the own
-// row), Ap = A p_k, partial p_k . Ap.
+// Ap = A p, partial p . Ap.
 template <typename R, int BLK> __global__ void __launch_bounds__(kBlock) k_ie_spmv(IeDev d) {
   __shared__ double lds[4];
   const IeCtrl *c = d.ctrl;
   if (c->done) return;
-  const int k = c->k;
-  const double beta = c->beta;
-  const double *__restrict__ pin = (k & 1) ? d.p0 : d.p1;  // p_{k-1} (k >= 1)
-  double *__restrict__ pout = (k & 1) ? d.p1 : d.p0;       // p_k
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int n = blk * kBlock + threadIdx.x;
   double pap = 0.0;
   if (n < d.n) {
     double y[3] = {0.0, 0.0, 0.0};
-    const double *__restrict__ z = d.z;
+    const double *__restrict__ p = d.p;
     row_apply<R, BLK>(d, n, [&](int col, double &x0, double &x1, double &x2) {
-      if (k == 0) {
-        x0 = pout[3 * (size_t)col];
-        x1 = pout[3 * (size_t)col + 1];
-        x2 = pout[3 * (size_t)col + 2];
-      } else {
-        x0 = z[3 * (size_t)col] + beta * pin[3 * (size_t)col];
-        x1 = z[3 * (size_t)col + 1] + beta * pin[3 * (size_t)col + 1];
-        x2 = z[3 * (size_t)col + 2] + beta * pin[3 * (size_t)col + 2];
-      }
+      x0 = p[3 * (size_t)col];
+      x1 = p[3 * (size_t)col + 1];
+      x2 = p[3 * (size_t)col + 2];
     }, y);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       const size_t i = 3 * (size_t)n + q;
-      const double pk = k == 0 ? pout[i] : d.z[i] + beta * pin[i];
-      if (k != 0) pout[i] = pk;
       d.Ap[i] = y[q];
-      pap += pk * y[q];
+      pap += p[i] * y[q];
     }
   }
   pap = block_sum256(pap, lds);
@@ -394,7 +381,7 @@ __global__ void __launch_bounds__(kBlock) k_ie_update(IeDev d) {
   const IeCtrl *c = d.ctrl;
   if (c->done) return;
   const double alpha = c->alpha;
-  const double *__restrict__ pk = (c->k & 1) ? d.p1 : d.p0;
+  const double *__restrict__ pk = d.p;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   double rz = 0.0, rr = 0.0;
   if (i < 3 * (size_t)d.n) {
@@ -428,6 +415,15 @@ __global__ void __launch_bounds__(1024) k_ie_beta(IeDev d, int nb) {
     c->converged = (rz <= d.tol2 * c->bb) ? 1 : 0;
     c->done = c->converged || c->k >= d.maxit;
   }
+}
+
+// p = z + beta p for the next iteration (after k_ie_beta), one thread per vector entry.
+__global__ void __launch_bounds__(kBlock) k_ie_pupdate(IeDev d) {
+  const IeCtrl *c = d.ctrl;
+  if (c->done) return;
+  const double beta = c->beta;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 3 * (size_t)d.n) d.p[i] = d.z[i] + beta * d.p[i];
 }
 
 // v = solution, x += h v for every node (cublasSaxpy in doBackEuler, CUDAInterface.cu:873).
@@ -623,6 +619,7 @@ template <typename R> void IeLaunch<R>::update(const IeDev &d, hipStream_t st) {
 }
 template <typename R> void IeLaunch<R>::beta(const IeDev &d, hipStream_t st) {
   hipLaunchKernelGGL(k_ie_beta, dim3(1), dim3(1024), 0, st, d, nb_of(3 * d.n));
+  hipLaunchKernelGGL(k_ie_pupdate, dim3(nb_of(3 * d.n)), dim3(kBlock), 0, st, d);
 }
 
 template <typename R>
