@@ -525,45 +525,59 @@ template <typename R> __device__ __forceinline__ void pd_rotation(const R *F, R 
     for (int j = 0; j < 3; ++j) Rot[3 * i + j] = s0 * U[3 * i] * V[3 * j] + U[3 * i + 1] * V[3 * j + 1] + s0 * U[3 * i + 2] * V[3 * j + 2];
 }
 
-// Local step (projectLocalConstraints2, ProjDynamic.cpp:223-308): per node, over its tets in
-// ascending order, the projection w_k Ac_k^T vec(R_k) restricted to its corner, i.e.
-// w_k R_k c_i; out = s + that sum (s may be null: the projection alone).  Positions are read
-// from pos (3N node-major doubles).
+// Local step (projectLocalConstraints2, ProjDynamic.cpp:223-308), in two passes like the
+// reference's: k_pd_tet evaluates each tet once -- F from the iterate pos (3N node-major
+// doubles), the SVD, R -- and writes its four corner projections w_k R_k c_i to rec (12 values
+// per tet); k_pd_gather sums each node's corner records over its tets in ascending order,
+// out = s + sum (s may be null: the projection alone).
 template <typename R>
 __global__ void __launch_bounds__(kBlock)
-    k_pd_local(DevParams p, float stiffness, int n_nodes, const double *__restrict__ pos, const double *__restrict__ srhs,
-               double *__restrict__ out, const int4 *__restrict__ tets, const float *__restrict__ dmi,
-               const float *__restrict__ vol, const int *__restrict__ inc_off, const int *__restrict__ inc) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= n_nodes) return;
+    k_pd_tet(DevParams p, float stiffness, const double *__restrict__ pos, const int4 *__restrict__ tets,
+             const float *__restrict__ dmi, const float *__restrict__ vol, R *__restrict__ rec) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int T = p.nt;
-  R acc[3] = {(R)0, (R)0, (R)0};
-  for (int e = inc_off[n]; e < inc_off[n + 1]; ++e) {
-    const int t = inc[e] >> 2, ci = inc[e] & 3;
-    const int4 q4 = tets[t];
-    const int qn[4] = {q4.x, q4.y, q4.z, q4.w};
-    R X[4][3];
+  if (t >= T) return;
+  const int4 q4 = tets[t];
+  const int qn[4] = {q4.x, q4.y, q4.z, q4.w};
+  R X[4][3];
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+  for (int c = 0; c < 4; ++c)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) X[c][k] = (R)pos[3 * (size_t)qn[c] + k];
-    R Dmi[9];
+    for (int k = 0; k < 3; ++k) X[c][k] = (R)pos[3 * (size_t)qn[c] + k];
+  R Dmi[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) Dmi[k] = (R)dmi[(size_t)k * T + t];
-    R Ds[9], F[9], Rot[9];
+  for (int k = 0; k < 9; ++k) Dmi[k] = (R)dmi[(size_t)k * T + t];
+  R Ds[9], F[9], Rot[9];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      Ds[3 * k + 0] = X[1][k] - X[0][k];
-      Ds[3 * k + 1] = X[2][k] - X[0][k];
-      Ds[3 * k + 2] = X[3][k] - X[0][k];
-    }
-    mul3(Ds, Dmi, F);
-    pd_rotation(F, Rot);
+  for (int k = 0; k < 3; ++k) {
+    Ds[3 * k + 0] = X[1][k] - X[0][k];
+    Ds[3 * k + 1] = X[2][k] - X[0][k];
+    Ds[3 * k + 2] = X[3][k] - X[0][k];
+  }
+  mul3(Ds, Dmi, F);
+  pd_rotation(F, Rot);
+  const R w = (R)stiffness * (R)vol[t];
+#pragma unroll
+  for (int ci = 0; ci < 4; ++ci) {
     R c[3];
     shape_vec(Dmi, ci, c);
-    const R w = (R)stiffness * (R)vol[t];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) acc[k] += w * (Rot[3 * k] * c[0] + Rot[3 * k + 1] * c[1] + Rot[3 * k + 2] * c[2]);
+    for (int k = 0; k < 3; ++k)
+      rec[12 * (size_t)t + 3 * ci + k] = w * (Rot[3 * k] * c[0] + Rot[3 * k + 1] * c[1] + Rot[3 * k + 2] * c[2]);
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_pd_gather(int n_nodes, const int *__restrict__ inc_off, const int *__restrict__ inc, const R *__restrict__ rec,
+                const double *__restrict__ srhs, double *__restrict__ out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n_nodes) return;
+  R acc[3] = {(R)0, (R)0, (R)0};
+  for (int e = inc_off[n]; e < inc_off[n + 1]; ++e) {
+    const R *r = rec + 3 * (size_t)inc[e];  // tet 4t + corner: record 12 t + 3 corner
+#pragma unroll
+    for (int k = 0; k < 3; ++k) acc[k] += r[k];
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -641,8 +655,11 @@ void IeLaunch<R>::pd_begin(const DevMesh<R> &m, const DevParams &p, const IeDev 
 template <typename R>
 void IeLaunch<R>::pd_local(const DevMesh<R> &m, const DevParams &p, float stiffness, const double *pos,
                            const double *srhs, double *out, hipStream_t st) {
-  hipLaunchKernelGGL(k_pd_local<R>, dim3(nb_of(p.nn)), dim3(kBlock), 0, st, p, stiffness, p.nn, pos, srhs, out, m.tets,
-                     m.dmi, m.vol, m.inc_off, m.inc);
+  // the per-tet force records of the standalone calls serve as the staging array
+  hipLaunchKernelGGL(k_pd_tet<R>, dim3(nb_of(p.nt)), dim3(kBlock), 0, st, p, stiffness, pos, m.tets, m.dmi, m.vol,
+                     m.frec);
+  hipLaunchKernelGGL(k_pd_gather<R>, dim3(nb_of(p.nn)), dim3(kBlock), 0, st, p.nn, m.inc_off, m.inc,
+                     (const R *)m.frec, srhs, out);
 }
 template <typename R>
 void IeLaunch<R>::pd_finish(const DevMesh<R> &m, const DevParams &p, float damping, const IeDev &d, hipStream_t st) {
