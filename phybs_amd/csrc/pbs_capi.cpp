@@ -657,9 +657,6 @@ template <typename R> static void launch_pass(pbs_solver *s, bool timed) {
   int i = timed ? timed_begin(s, K_TILE) : -1;
   Launch<R>::tile_pass(mesh<R>(s), s->td, s->dp, s->st);
   timed_end(s, i);
-  i = timed ? timed_begin(s, K_TILE_HD) : -1;
-  Launch<R>::tile_hd_pass(mesh<R>(s), s->td, s->dp, s->st);
-  timed_end(s, i);
   i = timed ? timed_begin(s, K_NODE) : -1;
   Launch<R>::node_pass(mesh<R>(s), s->td, s->dp, s->st);
   timed_end(s, i);
@@ -675,8 +672,7 @@ template <typename R> static int graph_build(pbs_solver *s, int G, int timed) {
   if (gs.exec) (void)hipGraphExecDestroy(gs.exec);
   gs.exec = nullptr;
   gs.passes = 0;
-  constexpr int KPP = 4;  // kernels per pass
-  while (timed && s->gev.size() < 2 * KPP * (size_t)G) {
+  while (timed && s->gev.size() < 6 * (size_t)G) {
     hipEvent_t e;
     HIPCHK(hipEventCreate(&e));
     s->gev.push_back(e);
@@ -684,15 +680,14 @@ template <typename R> static int graph_build(pbs_solver *s, int G, int timed) {
   HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
   hipError_t err = hipSuccess;
   for (int j = 0; j < G && err == hipSuccess; ++j)
-    for (int k = 0; k < KPP && err == hipSuccess; ++k) {
-      if (timed) err = hipEventRecordWithFlags(s->gev[2 * KPP * j + 2 * k], s->st, hipEventRecordExternal);
+    for (int k = 0; k < 3 && err == hipSuccess; ++k) {
+      if (timed) err = hipEventRecordWithFlags(s->gev[6 * j + 2 * k], s->st, hipEventRecordExternal);
       if (err != hipSuccess) break;
       if (k == 0) Launch<R>::tile_pass(mesh<R>(s), s->td, s->dp, s->st);
-      else if (k == 1) Launch<R>::tile_hd_pass(mesh<R>(s), s->td, s->dp, s->st);
-      else if (k == 2) Launch<R>::node_pass(mesh<R>(s), s->td, s->dp, s->st);
+      else if (k == 1) Launch<R>::node_pass(mesh<R>(s), s->td, s->dp, s->st);
       else Launch<R>::control(mesh<R>(s), s->td, s->dp, s->st);
       err = hipGetLastError();
-      if (timed && err == hipSuccess) err = hipEventRecordWithFlags(s->gev[2 * KPP * j + 2 * k + 1], s->st, hipEventRecordExternal);
+      if (timed && err == hipSuccess) err = hipEventRecordWithFlags(s->gev[6 * j + 2 * k + 1], s->st, hipEventRecordExternal);
     }
   hipGraph_t g = nullptr;
   hipError_t e2 = hipStreamEndCapture(s->st, &g);
@@ -716,11 +711,11 @@ template <typename R> static int graph_ready(pbs_solver *s, int G, int timed) {
 
 // Adds the launch times of the first `executed` passes of the last timed replay.
 static void graph_collect(pbs_solver *s, long executed) {
-  static const int kid[4] = {K_TILE, K_TILE_HD, K_NODE, K_CONTROL};
+  static const int kid[3] = {K_TILE, K_NODE, K_CONTROL};
   for (long j = 0; j < executed && j < s->graph[1].passes; ++j)
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 3; ++k) {
       float ms = 0.f;
-      if (hipEventElapsedTime(&ms, s->gev[8 * j + 2 * k], s->gev[8 * j + 2 * k + 1]) == hipSuccess) {
+      if (hipEventElapsedTime(&ms, s->gev[6 * j + 2 * k], s->gev[6 * j + 2 * k + 1]) == hipSuccess) {
         s->kt_ms[kid[k]] += ms;
         s->kt_n[kid[k]] += 1;
       }

@@ -64,10 +64,6 @@ struct TileDev {
 // Values per corner contribution in the tile pass's LDS: 4 floats (one 16-byte access) or 3 doubles.
 template <typename R> struct ContribWidth { static constexpr int value = 3; };
 template <> struct ContribWidth<float> { static constexpr int value = 4; };
-// Values per node position in the tile pass's LDS: 3 doubles (a 6-word stride spreads random node
-// reads over 16 bank pairs; 4 doubles, 8 words, would put them on 4) or 4 floats (one 16-byte read).
-template <typename R> struct PosStride { static constexpr int value = 3; };
-template <> struct PosStride<float> { static constexpr int value = 4; };
 
 template <typename R> struct Ring {
   R *b[4];  // each: 3 planes of nn values (x | y | z)
@@ -111,8 +107,6 @@ template <typename R> struct Launch {
                   hipStream_t st);
   static size_t tile_lds(const TileDev &td);
   static void tile_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
-  // Hessian diagonal of refresh passes (returns at once on the other passes)
-  static void tile_hd_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
   static void node_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
   static void grad_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
   static void control(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);

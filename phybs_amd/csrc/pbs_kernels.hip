@@ -259,9 +259,8 @@ template <typename R, int MAT>
 __device__ __forceinline__ void tile_tet(int phase, const uint4 lp, const R Dmi[9], R V, const R *__restrict__ sx,
                                          R *__restrict__ rec, R mu, R lam, int chk, int clamp, double &e_el) {
   constexpr int W = ContribWidth<R>::value;
-  constexpr int PS = PosStride<R>::value;
-  const R *p0 = sx + PS * (lp.x & 0xffff), *p1 = sx + PS * (lp.x >> 16), *p2 = sx + PS * (lp.y & 0xffff),
-          *p3 = sx + PS * (lp.y >> 16);
+  const R *p0 = sx + 4 * (lp.x & 0xffff), *p1 = sx + 4 * (lp.x >> 16), *p2 = sx + 4 * (lp.y & 0xffff),
+          *p3 = sx + 4 * (lp.y >> 16);
   R Ds[9];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -359,11 +358,7 @@ __device__ __forceinline__ void tile_gather(int phase, const TileInfo &ti, const
 //   LDS: positions of local nodes (4 R each), contributions (4 per tet, W values each).
 //   HBM per tet: 16 B of local ids / slots, 40 B of rest data from the SoA planes.
 // ---------------------------------------------------------------------------------------------
-// HDK = false: forces (and the energy on check passes) on every pass; HDK = true: the Hessian
-// diagonal, a separate launch that returns at once except on refresh passes, so that the common
-// force kernel is compiled without the registers of the stiffness terms (FP64: 94 instead of 126
-// VGPRs, one more wave per SIMD).
-template <typename R, int MAT, bool HDK>
+template <typename R, int MAT>
 __global__ void __launch_bounds__(256)
 k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, const float *__restrict__ dmi,
        const float *__restrict__ vol, R *__restrict__ fown, R *__restrict__ hpart, float *__restrict__ fownh,
@@ -374,33 +369,33 @@ k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
   const TileInfo ti = td.tiles[b];
   ctrl += ti.inst;
   if (ctrl->done) return;
-  if (HDK && !ctrl->hd_now) return;
-  const int chk = HDK ? 0 : ctrl->check_now;
-  const int phase = HDK ? 1 : 0;
+  const int hdnow = ctrl->hd_now, chk = ctrl->check_now;
   const R *__restrict__ x = ring.b[ctrl->cur];
   const int L = ti.own_cnt + ti.halo_cnt, N = p.nn, T = p.nt, tid = threadIdx.x;
   R *sx = (R *)smem;
-  constexpr int PS = PosStride<R>::value;
-  R *rec = sx + PS * td.lcap;
+  R *rec = sx + 4 * td.lcap;
   for (int i = tid; i < L; i += 256) {
     const int g = i < ti.own_cnt ? ti.own_start + i : td.halo[ti.halo_off + i - ti.own_cnt];
-    sx[PS * i] = x[g];
-    sx[PS * i + 1] = x[N + g];
-    sx[PS * i + 2] = x[2 * (size_t)N + g];
+    sx[4 * i] = x[g];
+    sx[4 * i + 1] = x[N + g];
+    sx[4 * i + 2] = x[2 * (size_t)N + g];
   }
   __syncthreads();
   const R mu = (R)td.inst_mu[ti.inst], lam = (R)td.inst_lam[ti.inst];
   double e_el = 0.0;
-  for (int s = tid; s < ti.tet_cnt; s += 256) {
-    const size_t t = (size_t)ti.tet_start + s;
-    const uint4 lp = td.tet_lp[t];
-    R Dmi[9];
+  for (int phase = 0; phase <= hdnow; ++phase) {
+    for (int s = tid; s < ti.tet_cnt; s += 256) {
+      const size_t t = (size_t)ti.tet_start + s;
+      const uint4 lp = td.tet_lp[t];
+      R Dmi[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) Dmi[k] = (R)dmi[k * (size_t)T + t];
-    tile_tet<R, MAT>(phase, lp, Dmi, (R)vol[t], sx, rec, mu, lam, chk, p.clamp, e_el);
+      for (int k = 0; k < 9; ++k) Dmi[k] = (R)dmi[k * (size_t)T + t];
+      tile_tet<R, MAT>(phase, lp, Dmi, (R)vol[t], sx, rec, mu, lam, chk, p.clamp, e_el);
+    }
+    __syncthreads();
+    tile_gather<R>(phase, ti, td, rec, N, fown, hpart, fownh, hparth);
+    if (phase == 0 && hdnow) __syncthreads();  // the Hessian phase reuses the contribution slots
   }
-  __syncthreads();
-  tile_gather<R>(phase, ti, td, rec, N, fown, hpart, fownh, hparth);
   if (chk) {
     const double se = block_sum(e_el, lds_red);
     if (tid == 0) td.tpart[b] = se;
@@ -694,27 +689,17 @@ void Launch<R>::tet(const DevMesh<R> &m, const TileDev &td, const DevParams &p, 
 }
 template <typename R>
 size_t Launch<R>::tile_lds(const TileDev &td) {
-  return ((size_t)PosStride<R>::value * td.lcap + 4 * (size_t)ContribWidth<R>::value * td.tcap) * sizeof(R);
+  return (4 * (size_t)td.lcap + 4 * (size_t)ContribWidth<R>::value * td.tcap) * sizeof(R);
 }
 template <typename R>
 void Launch<R>::tile_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
   const size_t lds = tile_lds(td);
   if (p.material == 0)
-    hipLaunchKernelGGL((k_tile<R, 0, false>), dim3(td.ntiles), dim3(256), lds, st, p, m.ctrl, m.ring, td, m.dmi, m.vol,
-                       m.fown, m.hpart, m.fownh, m.hparth);
+    hipLaunchKernelGGL((k_tile<R, 0>), dim3(td.ntiles), dim3(256), lds, st, p, m.ctrl, m.ring, td, m.dmi, m.vol, m.fown,
+                       m.hpart, m.fownh, m.hparth);
   else
-    hipLaunchKernelGGL((k_tile<R, 1, false>), dim3(td.ntiles), dim3(256), lds, st, p, m.ctrl, m.ring, td, m.dmi, m.vol,
-                       m.fown, m.hpart, m.fownh, m.hparth);
-}
-template <typename R>
-void Launch<R>::tile_hd_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
-  const size_t lds = tile_lds(td);
-  if (p.material == 0)
-    hipLaunchKernelGGL((k_tile<R, 0, true>), dim3(td.ntiles), dim3(256), lds, st, p, m.ctrl, m.ring, td, m.dmi, m.vol,
-                       m.fown, m.hpart, m.fownh, m.hparth);
-  else
-    hipLaunchKernelGGL((k_tile<R, 1, true>), dim3(td.ntiles), dim3(256), lds, st, p, m.ctrl, m.ring, td, m.dmi, m.vol,
-                       m.fown, m.hpart, m.fownh, m.hparth);
+    hipLaunchKernelGGL((k_tile<R, 1>), dim3(td.ntiles), dim3(256), lds, st, p, m.ctrl, m.ring, td, m.dmi, m.vol, m.fown,
+                       m.hpart, m.fownh, m.hparth);
 }
 template <typename R>
 void Launch<R>::node_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
