@@ -415,6 +415,8 @@ static int create_impl(int ninst, const int *node_off, const int *tet_off, const
   s->td.ntiles = s->plan.ntiles;
   s->td.lcap = std::max(s->plan.lcap, 1);
   s->td.tcap = std::max(s->plan.tcap, 1);
+  // one thread per tet of a full tile, in whole wavefronts, at most 256
+  s->td.threads = std::min(256, std::max(64, (s->plan.tile_tets + 63) / 64 * 64));
   s->td.halo_total = s->plan.halo_total;
   s->td.ninst = ninst;
   s->td.nblk = 0;

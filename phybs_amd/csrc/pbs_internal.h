@@ -43,7 +43,7 @@ struct Ctrl {
 
 // Tile partition on the device (see pbs_tiles.h for the host-side construction).
 struct TileDev {
-  int ntiles, lcap, tcap, pad;
+  int ntiles, lcap, tcap, threads;  // threads: workgroup size of the tile pass (64..256)
   // instances (independent meshes; 1 unless batched): node / tile / tet ranges, node blocks of
   // 256 nodes per instance (block b: instance blk_inst[b], first node blk_start[b]), Lame constants
   int ninst, nblk;

@@ -374,7 +374,8 @@ k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
   const int L = ti.own_cnt + ti.halo_cnt, N = p.nn, T = p.nt, tid = threadIdx.x;
   R *sx = (R *)smem;
   R *rec = sx + 4 * td.lcap;
-  for (int i = tid; i < L; i += 256) {
+  const int nthr = blockDim.x;
+  for (int i = tid; i < L; i += nthr) {
     const int g = i < ti.own_cnt ? ti.own_start + i : td.halo[ti.halo_off + i - ti.own_cnt];
     sx[4 * i] = x[g];
     sx[4 * i + 1] = x[N + g];
@@ -384,7 +385,7 @@ k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
   const R mu = (R)td.inst_mu[ti.inst], lam = (R)td.inst_lam[ti.inst];
   double e_el = 0.0;
   for (int phase = 0; phase <= hdnow; ++phase) {
-    for (int s = tid; s < ti.tet_cnt; s += 256) {
+    for (int s = tid; s < ti.tet_cnt; s += nthr) {
       const size_t t = (size_t)ti.tet_start + s;
       const uint4 lp = td.tet_lp[t];
       R Dmi[9];
@@ -695,10 +696,10 @@ template <typename R>
 void Launch<R>::tile_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
   const size_t lds = tile_lds(td);
   if (p.material == 0)
-    hipLaunchKernelGGL((k_tile<R, 0>), dim3(td.ntiles), dim3(256), lds, st, p, m.ctrl, m.ring, td, m.dmi, m.vol, m.fown,
+    hipLaunchKernelGGL((k_tile<R, 0>), dim3(td.ntiles), dim3(td.threads), lds, st, p, m.ctrl, m.ring, td, m.dmi, m.vol, m.fown,
                        m.hpart, m.fownh, m.hparth);
   else
-    hipLaunchKernelGGL((k_tile<R, 1>), dim3(td.ntiles), dim3(256), lds, st, p, m.ctrl, m.ring, td, m.dmi, m.vol, m.fown,
+    hipLaunchKernelGGL((k_tile<R, 1>), dim3(td.ntiles), dim3(td.threads), lds, st, p, m.ctrl, m.ring, td, m.dmi, m.vol, m.fown,
                        m.hpart, m.fownh, m.hparth);
 }
 template <typename R>
