@@ -115,7 +115,7 @@ def cpu_baseline_batch(inst, iters, grav, sample=8):
 
 # the learned correction is smoothed by 100 averaging sweeps and tried at 2, 1.5 and 1 times its
 # length (the best of the three, if better than the plain guess, starts the descent)
-WARM_OPTIONS = dict(sweeps=100, betas=(2.0, 1.5, 1.0))
+WARM_OPTIONS = dict(sweeps=100, betas=(1.5, 1.0))
 
 
 def impulse(rest, rng, k):
@@ -176,7 +176,7 @@ def warm_start_report(P, args, params, rest, tets, mask, x0, dims, plain_sps, pl
     """Learned warm start vs plain: on the benchmark bar (trained on another trajectory of the same
     bar) and on the 12x12x50 bar (43,200 tets)."""
     out = {}
-    model, info = train_warm_start(P, dims, params, args.train_steps, 7, 20, 8_000_000)
+    model, info = train_warm_start(P, dims, params, args.train_steps, 7, 30, 14_000_000)
     lw = timed_steps(P, rest, tets, mask, x0, params, args.warmup, args.steps, model)
     out["bench_bar"] = {"training": info, "plain_steps_per_s": plain_sps, "plain_iterations_per_step": plain_its,
                         "learned": lw, "speedup": lw["steps_per_s"] / plain_sps}
@@ -299,7 +299,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
     ap.add_argument("--no-implicit", action="store_true", help="skip the backward-Euler report")
-    ap.add_argument("--train-steps", type=int, default=6, help="training-trajectory steps on the benchmark bar")
+    ap.add_argument("--train-steps", type=int, default=12, help="training-trajectory steps on the benchmark bar")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) for counter calibration")
     args = ap.parse_args()
