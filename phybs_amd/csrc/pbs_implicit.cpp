@@ -307,7 +307,12 @@ template <typename R> int pd_get(pbs_solver *s, ImplicitState **out) {
     char *A = nullptr;
     rc = ialloc(S, &A, sizeof(R) * (size_t)S->slots);
     if (!rc) rc = ialloc(S, &S->pd_s, 3 * (size_t)s->nn);
+    if (!rc) rc = ialloc(S, &S->pd_inc_pos, 4 * (size_t)s->nt);
     if (rc) return rc;
+    std::vector<int> pos(4 * (size_t)s->nt);
+    const TilePlan &P = s->plan;
+    for (size_t e = 0; e < P.inc.size(); ++e) pos[P.inc[e]] = (int)e;
+    HIPCHK(hipMemcpy(S->pd_inc_pos, pos.data(), sizeof(int) * pos.size(), hipMemcpyHostToDevice));
     S->pd = S->dev;  // same layout, vectors and control block
     S->pd.A = A;
     S->pd.scalar = 1;
@@ -350,7 +355,7 @@ int pd_step_impl(pbs_solver *s, const pbs_pd_params *pp, const float *ext, float
   for (int it = 0; it < pp->iterations; ++it) {
     // local step at the current iterate (dev.x), global solve warm-started from it
     if (ea) HIPCHK(hipEventRecord(ea[0], s->st));
-    IeLaunch<R>::pd_local(m, s->dp, pp->stiffness, d.x, S->pd_s, d.b, s->st);
+    IeLaunch<R>::pd_local(m, s->dp, pp->stiffness, S->pd_inc_pos, d.x, S->pd_s, d.b, s->st);
     HIPCHK(hipGetLastError());
     if (ea) HIPCHK(hipEventRecord(ea[1], s->st));
     IeCtrl c{};
@@ -392,7 +397,7 @@ template <typename R> int pd_project_impl(pbs_solver *s, float stiffness, const 
   for (int i = 0; i < s->nn; ++i)
     for (int k = 0; k < 3; ++k) h[3 * (size_t)i + k] = (double)pos[3 * (size_t)perm[i] + k];
   HIPCHK(hipMemcpyAsync(S->pd.x, h.data(), sizeof(double) * n3, hipMemcpyHostToDevice, s->st));
-  IeLaunch<R>::pd_local(mesh<R>(s), s->dp, stiffness, S->pd.x, nullptr, S->pd.b, s->st);
+  IeLaunch<R>::pd_local(mesh<R>(s), s->dp, stiffness, S->pd_inc_pos, S->pd.x, nullptr, S->pd.b, s->st);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(h.data(), S->pd.b, sizeof(double) * n3, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));

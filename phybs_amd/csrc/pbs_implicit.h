@@ -55,8 +55,9 @@ template <typename R> struct IeLaunch {
   static void pd_matrix(const DevMesh<R> &m, const DevParams &p, float stiffness, const IeDev &d, hipStream_t st);
   static void pd_begin(const DevMesh<R> &m, const DevParams &p, const IeDev &d, double *srhs, int has_ext,
                        hipStream_t st);
-  static void pd_local(const DevMesh<R> &m, const DevParams &p, float stiffness, const double *pos, const double *srhs,
-                       double *out, hipStream_t st);
+  // inc_pos[4 t + corner] = position of (t, corner) in its node's incidence run
+  static void pd_local(const DevMesh<R> &m, const DevParams &p, float stiffness, const int *inc_pos, const double *pos,
+                       const double *srhs, double *out, hipStream_t st);
   static void pd_finish(const DevMesh<R> &m, const DevParams &p, float damping, const IeDev &d, hipStream_t st);
 };
 
@@ -72,6 +73,7 @@ struct ImplicitState {
   IeDev dev{};                                      // backward Euler: 3x3 blocks
   IeDev pd{};                                       // projective dynamics: same layout and vectors, scalar values
   double *pd_s = nullptr;                           // projective dynamics: momentum term (3N)
+  int *pd_inc_pos = nullptr;                        // projective dynamics: incidence slot of each (tet, corner)
   std::vector<int> slice_off_h, row_len_h, bcol_h;  // host copies (stiffness export)
   CgGraphs cg_ie, cg_pd;
   std::vector<hipEvent_t> ev;                       // timing events of the timed replays

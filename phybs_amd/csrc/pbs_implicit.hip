@@ -374,24 +374,30 @@ __global__ void __launch_bounds__(1024) k_ie_alpha(IeDev d, int nb) {
   }
 }
 
-// x += alpha p_k, r -= alpha Ap, z = D^-1 r; partials r.z, r.r.  One thread per vector entry
-// (3N of them), so that every load and store of a wavefront is one contiguous 512-byte access.
+// x += alpha p_k, r -= alpha Ap, z = D^-1 r; partials r.z, r.r.  A block covers kUpd = 1024
+// consecutive vector entries, 4 per thread 256 apart, so that every load and store of a wavefront
+// is one contiguous 512-byte access and there are 4x fewer partial sums to reduce.
+constexpr int kUpd = 4 * kBlock;
 __global__ void __launch_bounds__(kBlock) k_ie_update(IeDev d) {
   __shared__ double lds[4];
   const IeCtrl *c = d.ctrl;
   if (c->done) return;
   const double alpha = c->alpha;
   const double *__restrict__ pk = d.p;
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n3 = 3 * (size_t)d.n;
   double rz = 0.0, rr = 0.0;
-  if (i < 3 * (size_t)d.n) {
-    d.x[i] += alpha * pk[i];
-    const double r = d.r[i] - alpha * d.Ap[i];
-    const double z = d.dinv[i] * r;
-    d.r[i] = r;
-    d.z[i] = z;
-    rz = r * z;
-    rr = r * r;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const size_t i = (size_t)blockIdx.x * kUpd + j * kBlock + threadIdx.x;
+    if (i < n3) {
+      d.x[i] += alpha * pk[i];
+      const double r = d.r[i] - alpha * d.Ap[i];
+      const double z = d.dinv[i] * r;
+      d.r[i] = r;
+      d.z[i] = z;
+      rz += r * z;
+      rr += r * r;
+    }
   }
   rz = block_sum256(rz, lds);
   rr = block_sum256(rr, lds);
@@ -533,7 +539,8 @@ template <typename R> __device__ __forceinline__ void pd_rotation(const R *F, R 
 template <typename R>
 __global__ void __launch_bounds__(kBlock)
     k_pd_tet(DevParams p, float stiffness, const double *__restrict__ pos, const int4 *__restrict__ tets,
-             const float *__restrict__ dmi, const float *__restrict__ vol, R *__restrict__ rec) {
+             const float *__restrict__ dmi, const float *__restrict__ vol, const int *__restrict__ inc_pos,
+             R *__restrict__ rec) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int T = p.nt;
   if (t >= T) return;
@@ -561,21 +568,21 @@ __global__ void __launch_bounds__(kBlock)
   for (int ci = 0; ci < 4; ++ci) {
     R c[3];
     shape_vec(Dmi, ci, c);
+    R *o = rec + 3 * (size_t)inc_pos[4 * (size_t)t + ci];  // the (tet, corner) slot of its node's run
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-      rec[12 * (size_t)t + 3 * ci + k] = w * (Rot[3 * k] * c[0] + Rot[3 * k + 1] * c[1] + Rot[3 * k + 2] * c[2]);
+    for (int k = 0; k < 3; ++k) o[k] = w * (Rot[3 * k] * c[0] + Rot[3 * k + 1] * c[1] + Rot[3 * k + 2] * c[2]);
   }
 }
 
 template <typename R>
 __global__ void __launch_bounds__(kBlock)
-    k_pd_gather(int n_nodes, const int *__restrict__ inc_off, const int *__restrict__ inc, const R *__restrict__ rec,
+    k_pd_gather(int n_nodes, const int *__restrict__ inc_off, const R *__restrict__ rec,
                 const double *__restrict__ srhs, double *__restrict__ out) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= n_nodes) return;
   R acc[3] = {(R)0, (R)0, (R)0};
-  for (int e = inc_off[n]; e < inc_off[n + 1]; ++e) {
-    const R *r = rec + 3 * (size_t)inc[e];  // tet 4t + corner: record 12 t + 3 corner
+  for (int e = inc_off[n]; e < inc_off[n + 1]; ++e) {  // the node's run, in ascending tet order
+    const R *r = rec + 3 * (size_t)e;
 #pragma unroll
     for (int k = 0; k < 3; ++k) acc[k] += r[k];
   }
@@ -629,10 +636,10 @@ template <typename R> void IeLaunch<R>::alpha(const IeDev &d, hipStream_t st) {
   hipLaunchKernelGGL(k_ie_alpha, dim3(1), dim3(1024), 0, st, d, nb_of(d.n));
 }
 template <typename R> void IeLaunch<R>::update(const IeDev &d, hipStream_t st) {
-  hipLaunchKernelGGL(k_ie_update, dim3(nb_of(3 * d.n)), dim3(kBlock), 0, st, d);
+  hipLaunchKernelGGL(k_ie_update, dim3((3 * d.n + kUpd - 1) / kUpd), dim3(kBlock), 0, st, d);
 }
 template <typename R> void IeLaunch<R>::beta(const IeDev &d, hipStream_t st) {
-  hipLaunchKernelGGL(k_ie_beta, dim3(1), dim3(1024), 0, st, d, nb_of(3 * d.n));
+  hipLaunchKernelGGL(k_ie_beta, dim3(1), dim3(1024), 0, st, d, (3 * d.n + kUpd - 1) / kUpd);
   hipLaunchKernelGGL(k_ie_pupdate, dim3(nb_of(3 * d.n)), dim3(kBlock), 0, st, d);
 }
 
@@ -653,13 +660,14 @@ void IeLaunch<R>::pd_begin(const DevMesh<R> &m, const DevParams &p, const IeDev 
                      m.ext, has_ext, m.mask);
 }
 template <typename R>
-void IeLaunch<R>::pd_local(const DevMesh<R> &m, const DevParams &p, float stiffness, const double *pos,
-                           const double *srhs, double *out, hipStream_t st) {
-  // the per-tet force records of the standalone calls serve as the staging array
+void IeLaunch<R>::pd_local(const DevMesh<R> &m, const DevParams &p, float stiffness, const int *inc_pos,
+                           const double *pos, const double *srhs, double *out, hipStream_t st) {
+  // the per-tet force records of the standalone calls (12 R per tet) serve as the staging array,
+  // filled in incidence order
   hipLaunchKernelGGL(k_pd_tet<R>, dim3(nb_of(p.nt)), dim3(kBlock), 0, st, p, stiffness, pos, m.tets, m.dmi, m.vol,
-                     m.frec);
-  hipLaunchKernelGGL(k_pd_gather<R>, dim3(nb_of(p.nn)), dim3(kBlock), 0, st, p.nn, m.inc_off, m.inc,
-                     (const R *)m.frec, srhs, out);
+                     inc_pos, m.frec);
+  hipLaunchKernelGGL(k_pd_gather<R>, dim3(nb_of(p.nn)), dim3(kBlock), 0, st, p.nn, m.inc_off, (const R *)m.frec, srhs,
+                     out);
 }
 template <typename R>
 void IeLaunch<R>::pd_finish(const DevMesh<R> &m, const DevParams &p, float damping, const IeDev &d, hipStream_t st) {
