@@ -193,6 +193,32 @@ def warm_start_report(P, args, params, rest, tets, mask, x0, dims, plain_sps, pl
     return out
 
 
+def fixed_iteration_report(P, rest, tets, mask, grav, steps=10):
+    """The 6.3M-tet bar at 64 FP32 descent iterations per step from rest, for both materials
+    (neo-Hookean and the stretch-based energy): steps/s, descent iterations/s and the force kernel
+    against HBM bandwidth."""
+    out = {}
+    nt, nn = len(tets), len(rest)
+    for name, mat in (("neo_hookean", 0), ("stretch", 1)):
+        prm = P.default_params(dt=1.0 / 30.0, grav=grav, material=mat, iterations=64,
+                               profile=[(1, 0.96), (7, 0.991), (10, 0.9995)])
+        s = P.Solver(rest, tets, mask, prm)
+        s.step()
+        s.set_profiling(8)
+        s.reset_kernel_times()
+        t0 = time.perf_counter()
+        st = [s.step() for _ in range(steps)]
+        el = time.perf_counter() - t0
+        kt = s.kernel_times()
+        s.close()
+        alg = algorithmic_bytes(nt, nn, 4)
+        ach = alg["k_tile"] / (kt["k_tile"][0] * 1e-3) / 1e9
+        out[name] = {"steps_per_s": steps / el, "descent_iterations_per_s": sum(q.iterations for q in st) / el,
+                     "kernel_avg_ms": {k: kt[k][0] for k in ("k_tile", "k_node", "k_control")},
+                     "k_tile_achieved_gbs": ach, "k_tile_frac": ach / HBM_PEAK_GBS}
+    return out
+
+
 def implicit_report(P, rest, tets, mask, material, grav, steps=3):
     """Backward-Euler steps (the reference's GPU implicit path) on the benchmark bar in FP32 from
     rest under gravity: steps/s, CG iterations, and the SpMV kernel against HBM bandwidth.  SpMV
@@ -404,6 +430,8 @@ def main():
     if relg and not args.no_learned and not batch:
         out["learned_warm_start"] = warm_start_report(P, args, params, rest, tets, mask, x0, (nx, ny, nz),
                                                       steps_per_s, iters / args.steps)
+    if not batch and args.workload == "c2a":
+        out["fixed_64_iterations_f32"] = fixed_iteration_report(P, rest, tets, mask, grav)
     if not args.no_implicit and not batch:
         out["implicit_euler"] = implicit_report(P, rest, tets, mask, material, grav)
         out["projective_dynamics"] = pd_report(P, rest, tets, mask, grav)
