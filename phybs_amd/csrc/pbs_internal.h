@@ -137,11 +137,12 @@ template <typename R> struct LearnLaunch {
   // learned guess y + out * scale into ring.b[1] for free nodes (fixed nodes copy ring.b[0])
   static void guess(const DevMesh<R> &m, const DevParams &p, const MlpDev &net, const float *feat, const R *y,
                     hipStream_t st);
-  // correction d = ring.b[1] - ring.b[0], smoothed by iters sweeps (ring.b[2..3] hold d and
+  // correction d = ring.b[1] - ring.b[0], smoothed by iters sweeps in FP32 (sm: 2 x N float4
   // scratch), then the selection: the gradient norm of the plain guess ring.b[0] and of
   // ring.b[0] + beta d for each beta; the descent starts from the best of them
   static void select(const DevMesh<R> &m, const TileDev &td, const DevParams &p, const int *adj_off, const int *adj,
-                     int iters, const float *betas, int nbeta, hipStream_t st);
+                     const int *sell_off, const int *sell_col, int iters, const float *betas, int nbeta, float *sm,
+                     hipStream_t st);
   // appends (features, x_new - y) of free nodes to the sample buffers
   static void collect(const DevMesh<R> &m, const DevParams &p, const float *feat, const R *y, const int *free_rank,
                       long base, float *ds_x, float *ds_y, hipStream_t st);

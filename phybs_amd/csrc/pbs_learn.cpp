@@ -91,6 +91,24 @@ static int learn_get(pbs_solver *s, LearnState **out) {
   rc |= lalloc(L, &L->adj_off, N + 1);
   rc |= lalloc(L, &L->adj, std::max<size_t>(adj.size(), 1));
   rc |= lalloc(L, &L->free_rank, N);
+  rc |= lalloc(L, &L->smooth, 8 * N);
+  // the same adjacency as SELL-64 slices for the smoothing sweeps: slot j of row l of slice s at
+  // (sell_off[s] + j) * 64 + l, rows padded with the node itself to the slice's longest row
+  const size_t nsl = (N + 63) / 64;
+  std::vector<int> soff(nsl + 1, 0);
+  for (size_t sl = 0; sl < nsl; ++sl) {
+    int w = 0;
+    for (size_t n = 64 * sl; n < std::min(N, 64 * sl + 64); ++n) w = std::max(w, off[n + 1] - off[n]);
+    soff[sl + 1] = soff[sl] + w;
+  }
+  std::vector<int> scol(64 * (size_t)soff[nsl]);
+  for (size_t n = 0; n < N; ++n) {
+    const size_t g0 = soff[n >> 6], w = soff[(n >> 6) + 1] - g0, lane = n & 63;
+    for (size_t j = 0; j < w; ++j)
+      scol[(g0 + j) * 64 + lane] = (int)j < off[n + 1] - off[n] ? adj[off[n] + j] : (int)n;
+  }
+  rc |= lalloc(L, &L->sell_off, nsl + 1);
+  rc |= lalloc(L, &L->sell_col, std::max<size_t>(scol.size(), 1));
   if (rc) {
     learn_release(L);
     return PBS_ERR_HIP;
@@ -111,6 +129,9 @@ static int learn_get(pbs_solver *s, LearnState **out) {
   if (e == hipSuccess && !adj.empty())
     e = hipMemcpyAsync(L->adj, adj.data(), sizeof(int) * adj.size(), hipMemcpyHostToDevice, s->st);
   if (e == hipSuccess) e = hipMemcpyAsync(L->free_rank, rank.data(), sizeof(int) * N, hipMemcpyHostToDevice, s->st);
+  if (e == hipSuccess) e = hipMemcpyAsync(L->sell_off, soff.data(), sizeof(int) * soff.size(), hipMemcpyHostToDevice, s->st);
+  if (e == hipSuccess && !scol.empty())
+    e = hipMemcpyAsync(L->sell_col, scol.data(), sizeof(int) * scol.size(), hipMemcpyHostToDevice, s->st);
   if (e == hipSuccess) e = hipMemsetAsync(L->m1, 0, sizeof(float) * MLP_NPARAM, s->st);
   if (e == hipSuccess) e = hipMemsetAsync(L->m2, 0, sizeof(float) * MLP_NPARAM, s->st);
   if (e == hipSuccess) e = hipStreamSynchronize(s->st);
@@ -153,8 +174,8 @@ template <typename R> int learn_before_descent(pbs_solver *s) {
   LearnLaunch<R>::features(m, s->dp, L->adj_off, L->adj, L->feat, (R *)L->y, s->st);
   if (L->enabled && L->have_model) {
     LearnLaunch<R>::guess(m, s->dp, mlp_dev(*L), L->feat, (const R *)L->y, s->st);
-    LearnLaunch<R>::select(m, s->td, s->dp, L->adj_off, L->adj, L->smooth_iters, L->betas.data(), (int)L->betas.size(),
-                           s->st);
+    LearnLaunch<R>::select(m, s->td, s->dp, L->adj_off, L->adj, L->sell_off, L->sell_col, L->smooth_iters, L->betas.data(), (int)L->betas.size(),
+                           L->smooth, s->st);
   }
   HIPCHK(hipGetLastError());
   return PBS_OK;
@@ -421,8 +442,8 @@ template <typename R> static int guess_impl(pbs_solver *s, LearnState *L, double
   Launch<R>::step_init(m, s->td, s->dp, s->bs, s->st);
   LearnLaunch<R>::features(m, s->dp, L->adj_off, L->adj, L->feat, (R *)L->y, s->st);
   LearnLaunch<R>::guess(m, s->dp, mlp_dev(*L), L->feat, (const R *)L->y, s->st);
-  LearnLaunch<R>::select(m, s->td, s->dp, L->adj_off, L->adj, L->smooth_iters, L->betas.data(), (int)L->betas.size(),
-                         s->st);
+  LearnLaunch<R>::select(m, s->td, s->dp, L->adj_off, L->adj, L->sell_off, L->sell_col, L->smooth_iters, L->betas.data(), (int)L->betas.size(),
+                         L->smooth, s->st);
   HIPCHK(hipGetLastError());
   for (int which = 0; which < 2; ++which) {
     double *dst = which ? learned : plain;

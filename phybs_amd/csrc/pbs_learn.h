@@ -23,6 +23,8 @@ struct LearnState {
   long nfree = 0;
   // per-step buffers: features (N x 18) and the inertial target y (3 planes of R)
   float *feat = nullptr;
+  float *smooth = nullptr;  // 2 x N float4: the correction and its next sweep
+  int *sell_off = nullptr, *sell_col = nullptr;  // adjacency as SELL-64 slices (smoothing sweeps)
   void *y = nullptr;
   // collected samples
   bool collecting = false;
