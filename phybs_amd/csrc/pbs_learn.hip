@@ -249,7 +249,11 @@ __global__ void k_corr_diff(int N, const int *__restrict__ mask, const R *__rest
 __global__ void k_corr_smooth(int N, const int *__restrict__ mask, const int *__restrict__ adj_off,
                               const int *__restrict__ sell_off, const int *__restrict__ sell_col,
                               const float4 *__restrict__ d, float4 *__restrict__ dn) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  // XCD-contiguous blocks (block i runs on XCD i % 8): a node's neighbours, mostly in the same or
+  // the adjacent blocks in Morton order, stay in that XCD's L2
+  const int per = (gridDim.x + 7) >> 3, xcd = blockIdx.x & 7, full = gridDim.x - 8 * (per - 1);
+  const int blk = (xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1)) + (blockIdx.x >> 3);
+  const int n = blk * blockDim.x + threadIdx.x;
   if (n >= N) return;
   const int e0 = adj_off[n], e1 = adj_off[n + 1];
   const int g0 = sell_off[n >> 6], width = sell_off[(n >> 6) + 1] - g0, lane = n & 63;
