@@ -59,7 +59,8 @@ def algorithmic_bytes(nt, nn, rsz):
             = 56 B; per node the position read and the force written (2 x 3 rsz).
     k_node: per node the reads of x, x~, x_last and the force (4 x 3 rsz), Hessian diagonal
             (3 x FP32), mass (FP32) and mask (int32), and the write of x_next (3 rsz)."""
-    return {"k_tile": 56.0 * nt + 6.0 * rsz * nn, "k_node": (15.0 * rsz + 20.0) * nn}
+    t = 56.0 * nt + 6.0 * rsz * nn
+    return {"k_tile": t, "k_tile_mixed": t, "k_node": (15.0 * rsz + 20.0) * nn}
 
 
 def cpu_baseline(rest, tets, mask, material, iters_per_step, grav, seconds=15.0):
@@ -169,7 +170,7 @@ def timed_steps(P, rest, tets, mask, x0, params, warmup, steps, model=None):
     return {"steps_per_s": steps / dt, "iterations_per_step": float(np.mean([q.iterations for q in st])),
             "guess_used_fraction": float(np.mean([q.used_learned for q in st])),
             "g_learned_over_g_plain": [q.g_learned / q.g0_norm for q in st] if model is not None else None,
-            "converged_steps": int(sum(q.converged for q in st))}
+            "converged_steps": int(sum(q.converged for q in st)), "rollbacks": int(sum(q.rollbacks for q in st))}
 
 
 def warm_start_report(P, args, params, rest, tets, mask, x0, dims, plain_sps, plain_its):
@@ -321,6 +322,9 @@ def main():
     ap.add_argument("--precision", default="auto", choices=["auto", "f32", "f64"],
                     help="state and per-tet arithmetic (auto: f64 for rel-grad, f32 for fixed iterations)")
     ap.add_argument("--tol", type=float, default=1e-4)
+    ap.add_argument("--mixed-switch", type=float, default=0.0,
+                    help="FP64 rel-grad: FP32 per-tet arithmetic while |g|/|g0| is above this, FP64 below "
+                         "(0: FP64 throughout)")
     ap.add_argument("--max-iterations", type=int, default=40000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
@@ -367,10 +371,10 @@ def main():
     # of a pass would belong to finished instances and the per-launch roofline would be meaningless
     relg = args.mode == "relgrad" and not batch
     f64 = args.precision == "f64" or (args.precision == "auto" and relg)
-    params = P.default_params(
-        dt=1.0 / 30.0, grav=grav, material=material, precision=P.FP64 if f64 else P.FP32,
-        iterations=fixed_iters, relgrad_tol=args.tol if relg else 0.0, max_iterations=args.max_iterations,
-        profile=[(1, 0.96), (7, 0.991), (10, 0.9995)], tile_tets=-1 if batch else 0)
+    pkw = dict(dt=1.0 / 30.0, grav=grav, material=material, precision=P.FP64 if f64 else P.FP32,
+               iterations=fixed_iters, relgrad_tol=args.tol if relg else 0.0, max_iterations=args.max_iterations,
+               profile=[(1, 0.96), (7, 0.991), (10, 0.9995)], tile_tets=-1 if batch else 0)
+    params = P.default_params(**pkw, mixed_switch=args.mixed_switch if (relg and f64) else 0.0)
     if batch:
         s = P.Solver.batch([(i[0], i[1], i[2]) for i in inst], [i[4] for i in inst], [i[5] for i in inst], params)
     else:
@@ -391,7 +395,7 @@ def main():
     passes = sum(st.passes for st in stats)
     nt, nn = len(tets), len(rest)
     alg = algorithmic_bytes(nt, nn, 8 if f64 else 4)
-    dom = max(("k_tile", "k_node"), key=lambda k: kt[k][0] * kt[k][1])
+    dom = max((k for k in ("k_tile", "k_tile_mixed", "k_node") if k in kt), key=lambda k: kt[k][0] * kt[k][1])
     avg_ms = kt[dom][0]
     achieved = alg[dom] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     steps_per_s = args.steps / elapsed
@@ -406,14 +410,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if f64 else "f32",
+        "dtype": ("f64 state, f32 per-tet arithmetic above the switch" if params.mixed_switch > 0 else "f64")
+                 if f64 else "f32",
         "data": "synthetic",
         "config": {"workload": (f"c3: 256 bars (nx, ny 4-16, nz 8-64; E 1e4-1e6 Pa; nu 0.3-0.49) in one batched "
                                 f"solver, {nt} tets, {nn} nodes, " if batch else
                                 f"{args.workload}: {nx}x{ny}x{nz} bar, {nt} tets, {nn} nodes, ")
                                + f"{'stretch' if material else 'neo-Hookean'}, "
                                + (f"rel-grad {args.tol:g}" if relg else f"{fixed_iters} iterations/step"),
-                   "dt": 1.0 / 30.0, "tets": nt, "nodes": nn},
+                   "dt": 1.0 / 30.0, "tets": nt, "nodes": nn, "mixed_switch": params.mixed_switch},
         "iterations_per_step": iters / args.steps,
         "descent_iterations_per_s": iters / elapsed,
         "rollbacks": sum(st.rollbacks for st in stats),
@@ -429,6 +434,13 @@ def main():
         out["autotune_us_per_pass"] = s.autotune_report()
         out["tile_tets"] = s.get_params().tile_tets
     s.close()
+    if relg and f64 and not batch:
+        # the same steps with the other arithmetic: mixed mode (FP32 per-tet arithmetic down to
+        # |g|/|g0| = 2e-4) when the main run was FP64 throughout, and the reverse
+        sw = 0.0 if params.mixed_switch > 0 else 2e-4
+        r = timed_steps(P, rest, tets, mask, x0, P.default_params(**pkw, mixed_switch=sw), args.warmup, args.steps)
+        r["mixed_switch"] = sw
+        out["other_arithmetic"] = r
     if relg and not args.no_learned and not batch:
         out["learned_warm_start"] = warm_start_report(P, args, params, rest, tets, mask, x0, (nx, ny, nz),
                                                       steps_per_s, iters / args.steps)

@@ -73,6 +73,12 @@ typedef struct pbs_params {
                            LDS; -1 = autotuned at creation (64, 128, 256, 512 timed on 32 passes) */
   int graph_passes;     /* descent passes per captured HIP graph replay: 0 = 32, -1 = no graph
                            (kernels launched one by one) */
+  double mixed_switch;  /* PBS_FP64 with graphs only (no reference counterpart): while |g|/|g_0| of
+                           the last graph replay is above this, the tile pass forms F in FP64 and
+                           evaluates the per-tet forces in FP32 (neo-Hookean: from H = F - I) with
+                           FP32 contributions; energies stay FP64; then FP64 to the end of the step.
+                           0 = never (default).  A value above relgrad_tol keeps the final passes
+                           and the convergence test in FP64. */
 } pbs_params;
 
 typedef struct pbs_step_stats {

@@ -34,7 +34,7 @@ class Params(C.Structure):
         ("profile_n", C.c_int), ("profile_k", C.c_int * 8), ("profile_rho", C.c_float * 8),
         ("semantics", C.c_int), ("precision", C.c_int), ("material", C.c_int),
         ("iterations", C.c_int), ("relgrad_tol", C.c_double), ("max_iterations", C.c_int),
-        ("block_size", C.c_int), ("tile_tets", C.c_int), ("graph_passes", C.c_int),
+        ("block_size", C.c_int), ("tile_tets", C.c_int), ("graph_passes", C.c_int), ("mixed_switch", C.c_double),
     ]
 
 

@@ -45,6 +45,8 @@ static int validate_params(const pbs_params *p) {
   if (p->tile_tets != 0 && p->tile_tets != -1 && (p->tile_tets < 16 || p->tile_tets > 4096))
     return set_error(PBS_ERR_ARG, "tile_tets must be 0, -1 or in [16, 4096]");
   if (p->graph_passes < -1 || p->graph_passes > 1024) return set_error(PBS_ERR_ARG, "graph_passes must be in [-1, 1024]");
+  if (!(p->mixed_switch >= 0 && p->mixed_switch < 1)) return set_error(PBS_ERR_ARG, "mixed_switch must lie in [0, 1)");
+  if (p->mixed_switch > 0 && p->precision != PBS_FP64) return set_error(PBS_ERR_ARG, "mixed_switch needs PBS_FP64");
   return PBS_OK;
 }
 
@@ -79,6 +81,7 @@ static void make_devparams(pbs_solver *s) {
   d.has_ext = 0;
   d.iterations = p.relgrad_tol > 0 ? p.max_iterations : p.iterations;
   d.relgrad_tol = p.relgrad_tol;
+  d.f32math = 0;
   s->bs = p.block_size ? p.block_size : 256;
   s->ntb = (s->nt + s->bs - 1) / s->bs;
   s->nnb = (s->nn + s->bs - 1) / s->bs;
@@ -656,7 +659,7 @@ extern "C" int pbs_get_velocities(pbs_solver *s, float *v, float *vl) {
 
 // ---- the step ----
 template <typename R> static void launch_pass(pbs_solver *s, bool timed) {
-  int i = timed ? timed_begin(s, K_TILE) : -1;
+  int i = timed ? timed_begin(s, s->dp.f32math ? K_TILE_MIXED : K_TILE) : -1;
   Launch<R>::tile_pass(mesh<R>(s), s->td, s->dp, s->st);
   timed_end(s, i);
   i = timed ? timed_begin(s, K_NODE) : -1;
@@ -669,8 +672,9 @@ template <typename R> static void launch_pass(pbs_solver *s, bool timed) {
 
 // Captures G descent passes into a HIP graph; the timed variant brackets each kernel with
 // external event-record nodes so that launch times are measured inside the replays.
+// Cached graphs: [timed + 2 * f32math] (plain / timed replays, FP64 or mixed-mode FP32 arithmetic).
 template <typename R> static int graph_build(pbs_solver *s, int G, int timed) {
-  auto &gs = s->graph[timed];
+  auto &gs = s->graph[timed + 2 * (s->dp.f32math ? 1 : 0)];
   if (gs.exec) (void)hipGraphExecDestroy(gs.exec);
   gs.exec = nullptr;
   gs.passes = 0;
@@ -706,15 +710,15 @@ template <typename R> static int graph_build(pbs_solver *s, int G, int timed) {
 }
 
 template <typename R> static int graph_ready(pbs_solver *s, int G, int timed) {
-  const auto &gs = s->graph[timed];
+  const auto &gs = s->graph[timed + 2 * (s->dp.f32math ? 1 : 0)];
   if (gs.exec && gs.passes == G && std::memcmp(&gs.dp, &s->dp, sizeof(DevParams)) == 0) return PBS_OK;
   return graph_build<R>(s, G, timed);
 }
 
 // Adds the launch times of the first `executed` passes of the last timed replay.
 static void graph_collect(pbs_solver *s, long executed) {
-  static const int kid[3] = {K_TILE, K_NODE, K_CONTROL};
-  for (long j = 0; j < executed && j < s->graph[1].passes; ++j)
+  const int kid[3] = {s->dp.f32math ? K_TILE_MIXED : K_TILE, K_NODE, K_CONTROL};
+  for (long j = 0; j < executed && j < s->graph[1 + 2 * (s->dp.f32math ? 1 : 0)].passes; ++j)
     for (int k = 0; k < 3; ++k) {
       float ms = 0.f;
       if (hipEventElapsedTime(&ms, s->gev[6 * j + 2 * k], s->gev[6 * j + 2 * k + 1]) == hipSuccess) {
@@ -737,11 +741,7 @@ template <typename R> static int step_impl(pbs_solver *s, const float *ext, floa
   const int G = s->p.graph_passes == 0 ? 32 : s->p.graph_passes;
   const bool use_graph = G > 0;
   const int period = s->prof_period;
-  if (use_graph) {
-    int rc = graph_ready<R>(s, G, 0);
-    if (!rc && period) rc = graph_ready<R>(s, G, 1);
-    if (rc) return rc;
-  }
+  s->dp.f32math = 0;  // step start, warm start and eager passes: FP64 arithmetic
   HIPCHK(hipEventRecord(s->ev0, s->st));
   s->pass_ordinal = -1;
   int i = timed_begin(s, K_INIT);
@@ -776,8 +776,39 @@ template <typename R> static int step_impl(pbs_solver *s, const float *ext, floa
       if (!q.done) return false;
     return true;
   };
+  // mixed mode: FP32 per-tet arithmetic until the last replay's |g|/|g0| (largest over the active
+  // instances) is at or below mixed_switch, FP64 from then on.  A rollback during an FP32 replay
+  // means the FP32 forces no longer resolve the remaining gradient: the step goes on in FP64 from
+  // the restored iterate, with the step length the step started with.
+  const bool mixed = use_graph && relg && s->p.mixed_switch > 0 && sizeof(R) == 8;
+  bool fp64_phase = !mixed;
+  std::vector<float> alpha0;
+  std::vector<int> rb0;
+  if (mixed) {
+    int rc = read_ctrl();
+    if (rc) return rc;
+    for (const Ctrl &q : ch) {
+      alpha0.push_back(q.alpha);
+      rb0.push_back(q.rollbacks);
+    }
+  }
   while (true) {
     if (use_graph) {
+      if (!fp64_phase) {
+        double worst = -1.0;
+        bool known = true;
+        for (const Ctrl &q : ch)
+          if (!q.done) {
+            if (!(q.g0 > 0.0)) known = false;
+            else worst = std::max(worst, q.g / q.g0);
+          }
+        if (launched > 0 && known && worst <= s->p.mixed_switch) fp64_phase = true;
+      }
+      s->dp.f32math = fp64_phase ? 0 : 1;
+      int rc = graph_ready<R>(s, G, 0);
+      if (!rc && period) rc = graph_ready<R>(s, G, 1);
+      if (rc) return rc;
+      const int gi = 2 * s->dp.f32math;
       const long reps = (n + G - 1) / G;
       for (long r = 0; r < reps; ++r) {
         const bool timed = period && (s->prof_tick++ % period) == 0;
@@ -785,12 +816,12 @@ template <typename R> static int step_impl(pbs_solver *s, const float *ext, floa
           int rc = read_ctrl();
           if (rc) return rc;
           const int before = max_passes();
-          HIPCHK(hipGraphLaunch(s->graph[1].exec, s->st));
+          HIPCHK(hipGraphLaunch(s->graph[gi + 1].exec, s->st));
           rc = read_ctrl();
           if (rc) return rc;
           graph_collect(s, max_passes() - before);
         } else {
-          HIPCHK(hipGraphLaunch(s->graph[0].exec, s->st));
+          HIPCHK(hipGraphLaunch(s->graph[gi].exec, s->st));
         }
         launched += G;
       }
@@ -804,12 +835,26 @@ template <typename R> static int step_impl(pbs_solver *s, const float *ext, floa
     }
     int rc = read_ctrl();
     if (rc) return rc;
+    if (!fp64_phase) {
+      bool rolled = false;
+      for (size_t q = 0; q < ch.size(); ++q)
+        if (ch[q].rollbacks > rb0[q]) {
+          ch[q].alpha = alpha0[q];
+          rolled = true;
+        }
+      if (rolled) {
+        fp64_phase = true;
+        HIPCHK(hipMemcpyAsync(m.ctrl, ch.data(), sizeof(Ctrl) * ch.size(), hipMemcpyHostToDevice, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+      }
+    }
     if (all_done()) break;
     if (launched >= cap) return set_error(PBS_ERR_STATE, "descent loop did not terminate within the pass cap");
     n = 1;
     for (const Ctrl &q : ch)
       if (!q.done) n = std::max<long>(n, relg ? 32 : std::max(1, q.budget - q.k) + 1);
   }
+  s->dp.f32math = 0;
   s->pass_ordinal = -1;
   i = timed_begin(s, K_FINAL);
   Launch<R>::step_final(m, s->td, s->dp, s->st);

@@ -216,6 +216,63 @@ __device__ __forceinline__ void neo_tet(const R *F, const R *Dmi, R vol, R mu, R
   }
 }
 
+// Neo-Hookean forces from the displacement gradient H = F - I (F = I + H), free of the
+// cancellation F ~ I causes in FP32: with t = tr H, s2 = ((tr H)^2 - tr(H^2)) / 2 and
+// delta = J - 1 = t + s2 + det H, Cayley-Hamilton gives adj(I + H) = (1 + t + s2) I - (1 + t) H + H^2,
+// so E = F^-1 - I = (-det H I - (1 + t) H + H^2) / J and
+//   P = mu (H - E^T) + lambda ln(1 + delta) (I + E^T),
+// every term a product of small quantities.  Forces as neo_tet: G = -vol P Dm^-T.
+template <typename R>
+__device__ __forceinline__ void neo_tet_disp(const R *H, const R *Dmi, R vol, R mu, R lam, R *f12) {
+  const R t = H[0] + H[4] + H[8];
+  R H2[9];
+  mul3(H, H, H2);
+  const R s2 = (R)0.5 * (t * t - (H2[0] + H2[4] + H2[8]));
+  R Cf[9];
+  const R dH = adj3(H, Cf);
+  const R delta = t + s2 + dH;
+  const R iJ = (R)1 / ((R)1 + delta);
+  const R lj = (sizeof(R) == 4) ? (R)log1pf((float)delta) : (R)log1p((double)delta);
+  const R c1 = (R)1 + t;
+  R P[9];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      // E^T[a][c] = E[c][a]
+      const R et = ((c == a ? -dH : (R)0) - c1 * H[3 * c + a] + H2[3 * c + a]) * iJ;
+      P[3 * a + c] = mu * (H[3 * a + c] - et) + lam * lj * ((c == a ? (R)1 : (R)0) + et);
+    }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    R g0 = -vol * (P[3 * a] * Dmi[0] + P[3 * a + 1] * Dmi[1] + P[3 * a + 2] * Dmi[2]);
+    R g1 = -vol * (P[3 * a] * Dmi[3] + P[3 * a + 1] * Dmi[4] + P[3 * a + 2] * Dmi[5]);
+    R g2 = -vol * (P[3 * a] * Dmi[6] + P[3 * a + 1] * Dmi[7] + P[3 * a + 2] * Dmi[8]);
+    f12[3 + a] = g0;
+    f12[6 + a] = g1;
+    f12[9 + a] = g2;
+    f12[a] = -(g0 + g1 + g2);
+  }
+}
+
+// Energy alone (psi vol, as neo_tet / stretch_tet compute it), for the mixed mode's check passes.
+template <typename R, int MAT> __device__ __forceinline__ R tet_energy(const R *F, R vol, R mu, R lam) {
+  R C[9];
+  const R J = adj3(F, C);
+  if (MAT == 0) {
+    R I1 = (R)0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) I1 += F[i] * F[i];
+    const R lj = dlog(J);
+    return ((R)0.5 * mu * (I1 - (R)3) - mu * lj + (R)0.5 * lam * lj * lj) * vol;
+  } else {
+    R U[9], S[3], V[9];
+    svd3(F, U, S, V);
+    const R e = (S[0] - (R)1) * (S[0] - (R)1) + (S[1] - (R)1) * (S[1] - (R)1) + (S[2] - (R)1) * (S[2] - (R)1);
+    return (mu * e + (R)0.5 * lam * (J - (R)1) * (J - (R)1)) * vol;
+  }
+}
+
 // Stretch material psi = sum mu (s_i - 1)^2 + lambda/2 (J - 1)^2 on signed singular values.
 //   P = 2 mu (F - U V^T) + lambda (J - 1) J F^-T.
 //   K(n,d) = vol [ 2 mu (|c_n|^2 - sum_{k<l} (a_kl . c_n)^2 / (s_k + s_l)) + lambda J^2 (c_n . w_d)^2 ]

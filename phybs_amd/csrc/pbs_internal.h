@@ -26,6 +26,7 @@ struct DevParams {
   int has_ext;
   int iterations;
   double relgrad_tol;
+  int f32math;  // FP64 state: the tile pass evaluates the per-tet terms in FP32 (mixed mode)
 };
 
 // Device-resident loop state.  The reference keeps k, alpha, the energies and the Chebyshev

@@ -12,6 +12,8 @@
 // vectors are 3 planes (x | y | z) of nn values (SoA); per-tet rest data are 10 planes (Dm^-1
 // entries, volume) plus 16 bytes per tet of tile-local ids and contribution slots for the tile
 // pass; per-tet force records of the standalone calls are 12 consecutive values.
+#include <type_traits>
+
 #include "pbs_device.h"
 #include "pbs_internal.h"
 
@@ -192,7 +194,8 @@ __device__ void control_apply(const DevParams &p, Ctrl &C, double E, double g2) 
     if (C.g0 < 0.0) C.g0 = g;
     C.first = 0;
   }
-  if (p.relgrad_tol > 0.0 && g <= p.relgrad_tol * C.g0) {
+  // (mixed mode: convergence is only declared on a pass with FP64 forces)
+  if (p.relgrad_tol > 0.0 && !p.f32math && g <= p.relgrad_tol * C.g0) {
     C.converged = 1;
     C.done = 1;
     C.passes++;
@@ -254,60 +257,85 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblk) {
 // One tet of a tile: F from the tile's LDS positions (stride 4 R per local node), the material
 // response, and its four corner contributions written to their slots of the LDS array rec (W
 // values each): forces on phase 0 (plus the clamped energy on check passes), Hessian-diagonal
-// entries rounded to FP32 on phase 1.
-template <typename R, int MAT>
-__device__ __forceinline__ void tile_tet(int phase, const uint4 lp, const R Dmi[9], R V, const R *__restrict__ sx,
-                                         R *__restrict__ rec, R mu, R lam, int chk, int clamp, double &e_el) {
-  constexpr int W = ContribWidth<R>::value;
+// entries rounded to FP32 on phase 1.  The arithmetic type C is R, or FP32 on FP64 positions (the
+// mixed mode).  Mixed mode: F is formed in R; the energy of check passes is evaluated in R from
+// that F (the same value the FP64 passes compute, so the line search compares like with like);
+// the neo-Hookean force is evaluated in FP32 from the displacement gradient H = F - I, whose FP32
+// rounding is relative to the strain instead of to 1; contributions are stored as FP32.
+template <typename R, typename C, int MAT>
+__device__ __forceinline__ void tile_tet(int phase, const uint4 lp, const float Dmf[9], float Vf,
+                                         const R *__restrict__ sx, C *__restrict__ rec, C mu, C lam, int chk, int clamp,
+                                         double &e_el) {
+  constexpr int W = ContribWidth<C>::value;
+  constexpr bool MIX = sizeof(C) < sizeof(R);
   const R *p0 = sx + 4 * (lp.x & 0xffff), *p1 = sx + 4 * (lp.x >> 16), *p2 = sx + 4 * (lp.y & 0xffff),
           *p3 = sx + 4 * (lp.y >> 16);
-  R Ds[9];
+  typedef typename std::conditional<MIX, R, C>::type FT;  // type F is formed in
+  FT Ds[9], Dm[9], Fw[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Dm[k] = (FT)Dmf[k];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const R x0 = p0[k];
-    Ds[3 * k + 0] = p1[k] - x0;
-    Ds[3 * k + 1] = p2[k] - x0;
-    Ds[3 * k + 2] = p3[k] - x0;
+    Ds[3 * k + 0] = (FT)(p1[k] - x0);
+    Ds[3 * k + 1] = (FT)(p2[k] - x0);
+    Ds[3 * k + 2] = (FT)(p3[k] - x0);
   }
-  R F[9];
-  mul3(Ds, Dmi, F);
-  R f12[12], h12[12], en;
+  mul3(Ds, Dm, Fw);
+  C F[9], Dmi[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    Dmi[k] = (C)Dmf[k];
+    F[k] = (C)Fw[k];
+  }
+  const C V = (C)Vf;
+  C f12[12], h12[12], en = (C)0;
   const unsigned q[4] = {lp.z & 0xffff, lp.z >> 16, lp.w & 0xffff, lp.w >> 16};
   if (phase == 0) {
-    if (MAT == 0) neo_tet<R, false>(F, Dmi, V, mu, lam, f12, en, h12);
-    else stretch_tet<R, false>(F, Dmi, V, mu, lam, f12, en, h12);
+    if (MIX && MAT == 0) {
+      C H[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) H[k] = (C)(Fw[k] - ((k & 3) == 0 ? (FT)1 : (FT)0));
+      neo_tet_disp<C>(H, Dmi, V, mu, lam, f12);
+    } else if (MAT == 0) {
+      neo_tet<C, false>(F, Dmi, V, mu, lam, f12, en, h12);
+    } else {
+      stretch_tet<C, false>(F, Dmi, V, mu, lam, f12, en, h12);
+    }
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      R *r = rec + W * q[c];
+      C *r = rec + W * q[c];
       r[0] = f12[3 * c];
       r[1] = f12[3 * c + 1];
       r[2] = f12[3 * c + 2];
     }
     if (chk) {
-      if (clamp == 1) en = en > (R)0 ? en : (R)0;          // CUDAInterface.cu:549
-      else if (clamp == 2) en = en > (R)1e-6 ? en : (R)0;  // IsotropicNeohookeanMaterial.cpp:129
-      e_el += (double)en;
+      R er = MIX ? tet_energy<R, MAT>((const R *)Fw, (R)Vf, (R)mu, (R)lam) : (R)en;
+      if (clamp == 1) er = er > (R)0 ? er : (R)0;          // CUDAInterface.cu:549
+      else if (clamp == 2) er = er > (R)1e-6 ? er : (R)0;  // IsotropicNeohookeanMaterial.cpp:129
+      e_el += (double)er;
     }
   } else {
-    if (MAT == 0) neo_tet<R, true>(F, Dmi, V, mu, lam, f12, en, h12);
-    else stretch_tet<R, true>(F, Dmi, V, mu, lam, f12, en, h12);
+    if (MAT == 0) neo_tet<C, true>(F, Dmi, V, mu, lam, f12, en, h12);
+    else stretch_tet<C, true>(F, Dmi, V, mu, lam, f12, en, h12);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      R *r = rec + W * q[c];
-      r[0] = (R)(float)h12[3 * c];
-      r[1] = (R)(float)h12[3 * c + 1];
-      r[2] = (R)(float)h12[3 * c + 2];
+      C *r = rec + W * q[c];
+      r[0] = (C)(float)h12[3 * c];
+      r[1] = (C)(float)h12[3 * c + 1];
+      r[2] = (C)(float)h12[3 * c + 2];
     }
   }
 }
 
 // Gather of one tile: each thread sums the contribution runs of its local nodes in order and
-// writes one partial per owned node (3 planes of N) and per halo node (xyz at its slot).
-template <typename R>
-__device__ __forceinline__ void tile_gather(int phase, const TileInfo &ti, const TileDev &td, const R *__restrict__ rec,
+// writes one partial per owned node (3 planes of N) and per halo node (xyz at its slot); the sums
+// are formed in R whatever the contributions' type Q.
+template <typename R, typename Q>
+__device__ __forceinline__ void tile_gather(int phase, const TileInfo &ti, const TileDev &td, const Q *__restrict__ rec,
                                             int N, R *__restrict__ fown, R *__restrict__ hpart,
                                             float *__restrict__ fownh, float *__restrict__ hparth) {
-  constexpr int W = ContribWidth<R>::value;
+  constexpr int W = ContribWidth<Q>::value;
   const int L = ti.own_cnt + ti.halo_cnt;
   const unsigned short *off = td.tile_off + ti.off_start;
   for (int i = threadIdx.x; i < L; i += blockDim.x) {
@@ -323,10 +351,10 @@ __device__ __forceinline__ void tile_gather(int phase, const TileInfo &ti, const
     if (phase == 0) {
       R a0 = (R)0, a1 = (R)0, a2 = (R)0;
       for (int j = o0; j < o1; ++j) {
-        const R *r = rec + W * j;
-        a0 += r[0];
-        a1 += r[1];
-        a2 += r[2];
+        const Q *r = rec + W * j;
+        a0 += (R)r[0];
+        a1 += (R)r[1];
+        a2 += (R)r[2];
       }
       R *o = i < ti.own_cnt ? fown : hpart;
       o[d0] = a0;
@@ -335,7 +363,7 @@ __device__ __forceinline__ void tile_gather(int phase, const TileInfo &ti, const
     } else {
       float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
       for (int j = o0; j < o1; ++j) {
-        const R *r = rec + W * j;
+        const Q *r = rec + W * j;
         a0 += (float)r[0];
         a1 += (float)r[1];
         a2 += (float)r[2];
@@ -358,7 +386,8 @@ __device__ __forceinline__ void tile_gather(int phase, const TileInfo &ti, const
 //   LDS: positions of local nodes (4 R each), contributions (4 per tet, W values each).
 //   HBM per tet: 16 B of local ids / slots, 40 B of rest data from the SoA planes.
 // ---------------------------------------------------------------------------------------------
-template <typename R, int MAT>
+// F32M (FP64 state only): the per-tet arithmetic in FP32 after F (the mixed mode's early passes).
+template <typename R, int MAT, bool F32M>
 __global__ void __launch_bounds__(256)
 k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, const float *__restrict__ dmi,
        const float *__restrict__ vol, R *__restrict__ fown, R *__restrict__ hpart, float *__restrict__ fownh,
@@ -372,8 +401,9 @@ k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
   const int hdnow = ctrl->hd_now, chk = ctrl->check_now;
   const R *__restrict__ x = ring.b[ctrl->cur];
   const int L = ti.own_cnt + ti.halo_cnt, N = p.nn, T = p.nt, tid = threadIdx.x;
+  typedef typename std::conditional<F32M, float, R>::type C;
   R *sx = (R *)smem;
-  R *rec = sx + 4 * td.lcap;
+  C *rec = (C *)(sx + 4 * td.lcap);
   const int nthr = blockDim.x;
   for (int i = tid; i < L; i += nthr) {
     const int g = i < ti.own_cnt ? ti.own_start + i : td.halo[ti.halo_off + i - ti.own_cnt];
@@ -382,16 +412,16 @@ k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
     sx[4 * i + 2] = x[2 * (size_t)N + g];
   }
   __syncthreads();
-  const R mu = (R)td.inst_mu[ti.inst], lam = (R)td.inst_lam[ti.inst];
+  const C mu = (C)td.inst_mu[ti.inst], lam = (C)td.inst_lam[ti.inst];
   double e_el = 0.0;
   for (int phase = 0; phase <= hdnow; ++phase) {
     for (int s = tid; s < ti.tet_cnt; s += nthr) {
       const size_t t = (size_t)ti.tet_start + s;
       const uint4 lp = td.tet_lp[t];
-      R Dmi[9];
+      float Dmf[9];
 #pragma unroll
-      for (int k = 0; k < 9; ++k) Dmi[k] = (R)dmi[k * (size_t)T + t];
-      tile_tet<R, MAT>(phase, lp, Dmi, (R)vol[t], sx, rec, mu, lam, chk, p.clamp, e_el);
+      for (int k = 0; k < 9; ++k) Dmf[k] = dmi[k * (size_t)T + t];
+      tile_tet<R, C, MAT>(phase, lp, Dmf, vol[t], sx, rec, mu, lam, chk, p.clamp, e_el);
     }
     __syncthreads();
     tile_gather<R>(phase, ti, td, rec, N, fown, hpart, fownh, hparth);
@@ -695,12 +725,19 @@ size_t Launch<R>::tile_lds(const TileDev &td) {
 template <typename R>
 void Launch<R>::tile_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
   const size_t lds = tile_lds(td);
-  if (p.material == 0)
-    hipLaunchKernelGGL((k_tile<R, 0>), dim3(td.ntiles), dim3(td.threads), lds, st, p, m.ctrl, m.ring, td, m.dmi, m.vol, m.fown,
-                       m.hpart, m.fownh, m.hparth);
+  const bool f32m = sizeof(R) == 8 && p.f32math;
+  if (p.material == 0 && !f32m)
+    hipLaunchKernelGGL((k_tile<R, 0, false>), dim3(td.ntiles), dim3(td.threads), lds, st, p, m.ctrl, m.ring, td, m.dmi,
+                       m.vol, m.fown, m.hpart, m.fownh, m.hparth);
+  else if (p.material == 0)
+    hipLaunchKernelGGL((k_tile<R, 0, true>), dim3(td.ntiles), dim3(td.threads), lds, st, p, m.ctrl, m.ring, td, m.dmi,
+                       m.vol, m.fown, m.hpart, m.fownh, m.hparth);
+  else if (!f32m)
+    hipLaunchKernelGGL((k_tile<R, 1, false>), dim3(td.ntiles), dim3(td.threads), lds, st, p, m.ctrl, m.ring, td, m.dmi,
+                       m.vol, m.fown, m.hpart, m.fownh, m.hparth);
   else
-    hipLaunchKernelGGL((k_tile<R, 1>), dim3(td.ntiles), dim3(td.threads), lds, st, p, m.ctrl, m.ring, td, m.dmi, m.vol, m.fown,
-                       m.hpart, m.fownh, m.hparth);
+    hipLaunchKernelGGL((k_tile<R, 1, true>), dim3(td.ntiles), dim3(td.threads), lds, st, p, m.ctrl, m.ring, td, m.dmi,
+                       m.vol, m.fown, m.hpart, m.fownh, m.hparth);
 }
 template <typename R>
 void Launch<R>::node_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {

@@ -28,11 +28,11 @@ using namespace pbs;
   } while (0)
 
 enum KernelId {
-  K_TILE = 0, K_NODE, K_CONTROL, K_INIT, K_FINAL, K_LEARN, K_IE_ASSEMBLE, K_IE_SPMV, K_IE_UPDATE, K_PD_LOCAL, K_COUNT
+  K_TILE = 0, K_NODE, K_CONTROL, K_INIT, K_FINAL, K_LEARN, K_IE_ASSEMBLE, K_IE_SPMV, K_IE_UPDATE, K_PD_LOCAL, K_TILE_MIXED, K_COUNT
 };
 static const char *const kKernelNames[K_COUNT] = {"k_tile",        "k_node",    "k_control",   "k_step_init",
                                                   "k_step_final",  "k_warm_start", "k_ie_assemble", "k_ie_spmv",
-                                                  "k_ie_update",   "k_pd_local"};
+                                                  "k_ie_update",   "k_pd_local",   "k_tile_mixed"};
 
 struct pbs_solver {
   int nn = 0, nt = 0;
@@ -70,7 +70,7 @@ struct pbs_solver {
     hipGraphExec_t exec = nullptr;
     DevParams dp{};  // parameters baked into the graph
     int passes = 0;
-  } graph[2];
+  } graph[4];
   std::vector<hipEvent_t> gev;
   int prof_period = 0;  // 0: no timing; n: time every n-th graph replay (or pass without graphs)
   long prof_tick = 0;

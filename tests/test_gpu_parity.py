@@ -259,6 +259,34 @@ def test_relgrad_mode_converges_like_oracle():
     assert np.abs(xg - xo).max() <= 1e-6
 
 
+@pytest.mark.parametrize("switch", [1e-3, 2e-4])
+def test_mixed_mode_converges_like_fp64_oracle(switch):
+    """FP64 state with FP32 per-tet arithmetic until |g|/|g0| <= switch, then FP64: every step
+    still reaches rel-grad 1e-4 (judged in FP64), the positions stay within the rel-grad mode's
+    own bound of the FP64 oracle, and the iteration count stays close to the FP64 one.  (A
+    rollback during the FP32 passes sends the step on in FP64 with a restarted Chebyshev
+    schedule, so single steps may take a different number of iterations.)"""
+    rest, tets, mask, x0 = bar_case()
+    say("mixed mode, switch", switch)
+    gp = P.default_params(precision=P.FP64, relgrad_tol=1e-4, max_iterations=5000, mixed_switch=switch)
+    op = O.default_params(relgrad_tol=1e-4, iterations=5000)
+    xg, xo, gst, ost = run_pair(rest, tets, mask, x0, 3, gp, op, 64)
+    for g, q in zip(gst, ost):
+        say("iters mixed", g.iterations, "oracle", q.iterations, "rollbacks", g.rollbacks, "g/g0", g.g_norm / g.g0_norm)
+        assert g.converged == 1 and g.g_norm <= 1e-4 * g.g0_norm
+    im, io = sum(g.iterations for g in gst), sum(q.iterations for q in ost)
+    assert abs(im - io) <= 0.25 * io
+    err = np.abs(xg - xo).max()
+    say("max |x_mixed - x_oracle|", err)
+    assert err <= 1e-6
+
+
+def test_mixed_mode_rejects_fp32_state():
+    rest, tets, mask, _ = bar_case(1, 1, 2, twist=0.0, jitter=0.0)
+    with pytest.raises(P.PhybsError):
+        P.Solver(rest, tets, mask, P.default_params(precision=P.FP32, mixed_switch=1e-3))
+
+
 def test_stretch_material_steps():
     rest, tets, mask, x0 = bar_case(twist=30.0)
     say("stretch material 20 steps")
