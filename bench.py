@@ -8,7 +8,7 @@ In rel-grad mode the line also reports the learned warm start against the plain 
 network is trained on the GPU on another seeded trajectory of the same bar (and of the
 12x12x50 bar, 43,200 tets, for the second comparison) and then steps the benchmark trajectory.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2a|c2b|c1|c0] [--mode relgrad|fixed]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload bar64x64x256|bar64x64x256-stretch|bar12x12x50|bar4x4x16|batch256] [--mode relgrad|fixed]
                   [--no-learned] [--no-cpu-baseline]
 """
 import argparse
@@ -26,12 +26,12 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 
 WORKLOADS = {
     # name: (nx, ny, nz, material, twist_deg, jitter, fixed iterations)
-    "c0": (4, 4, 16, 0, 0.0, 0.0, 96),
-    "c1": (12, 12, 50, 0, 90.0, 0.01, 96),
-    "c2a": (64, 64, 256, 0, 0.0, 0.0, 64),
-    "c2b": (64, 64, 256, 1, 0.0, 0.0, 64),
+    "bar4x4x16": (4, 4, 16, 0, 0.0, 0.0, 96),
+    "bar12x12x50": (12, 12, 50, 0, 90.0, 0.01, 96),
+    "bar64x64x256": (64, 64, 256, 0, 0.0, 0.0, 64),
+    "bar64x64x256-stretch": (64, 64, 256, 1, 0.0, 0.0, 64),
     # 256 independent bars in one batched solver (sizes and materials from batch_instances)
-    "c3": (0, 0, 0, 0, 0.0, 0.0, 96),
+    "batch256": (0, 0, 0, 0, 0.0, 0.0, 96),
 }
 
 
@@ -181,11 +181,11 @@ def warm_start_report(P, args, params, rest, tets, mask, x0, dims, plain_sps, pl
     lw = timed_steps(P, rest, tets, mask, x0, params, args.warmup, args.steps, model)
     out["bench_bar"] = {"training": info, "plain_steps_per_s": plain_sps, "plain_iterations_per_step": plain_its,
                         "learned": lw, "speedup": lw["steps_per_s"] / plain_sps}
-    c1 = (12, 12, 50)
-    r1, t1 = P.mesh_bar(*c1, 0.05)
+    d1 = (12, 12, 50)
+    r1, t1 = P.mesh_bar(*d1, 0.05)
     m1 = (r1[:, 0] == 0).astype(np.int32)
-    x1 = P.mesh_twist(r1, *c1, 0.05, 90.0, 0.01, 11)
-    model1, info1 = train_warm_start(P, c1, params, 120, 1, 30, 2_000_000)
+    x1 = P.mesh_twist(r1, *d1, 0.05, 90.0, 0.01, 11)
+    model1, info1 = train_warm_start(P, d1, params, 120, 1, 30, 2_000_000)
     p1 = timed_steps(P, r1, t1, m1, x1, params, 1, 30)
     l1 = timed_steps(P, r1, t1, m1, x1, params, 1, 30, model1)
     out["bar_12x12x50"] = {"tets": len(t1), "training": info1, "plain": p1, "learned": l1,
@@ -317,7 +317,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2a", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="bar64x64x256", choices=sorted(WORKLOADS))
     ap.add_argument("--mode", default="relgrad", choices=["relgrad", "fixed"])
     ap.add_argument("--precision", default="auto", choices=["auto", "f32", "f64"],
                     help="state and per-tet arithmetic (auto: f64 for rel-grad, f32 for fixed iterations)")
@@ -355,7 +355,7 @@ def main():
 
     nx, ny, nz, material, twist, jitter, fixed_iters = WORKLOADS[args.workload]
     edge = 0.05
-    batch = args.workload == "c3"
+    batch = args.workload == "batch256"
     if batch:
         inst = batch_instances(P)
         rest = np.concatenate([i[0] for i in inst])
@@ -413,7 +413,7 @@ def main():
         "dtype": ("f64 state, f32 per-tet arithmetic above the switch" if params.mixed_switch > 0 else "f64")
                  if f64 else "f32",
         "data": "synthetic",
-        "config": {"workload": (f"c3: 256 bars (nx, ny 4-16, nz 8-64; E 1e4-1e6 Pa; nu 0.3-0.49) in one batched "
+        "config": {"workload": (f"256 bars (nx, ny 4-16, nz 8-64; E 1e4-1e6 Pa; nu 0.3-0.49) in one batched "
                                 f"solver, {nt} tets, {nn} nodes, " if batch else
                                 f"{args.workload}: {nx}x{ny}x{nz} bar, {nt} tets, {nn} nodes, ")
                                + f"{'stretch' if material else 'neo-Hookean'}, "
@@ -444,7 +444,7 @@ def main():
     if relg and not args.no_learned and not batch:
         out["learned_warm_start"] = warm_start_report(P, args, params, rest, tets, mask, x0, (nx, ny, nz),
                                                       steps_per_s, iters / args.steps)
-    if not batch and args.workload == "c2a":
+    if not batch and args.workload == "bar64x64x256":
         out["fixed_64_iterations_f32"] = fixed_iteration_report(P, rest, tets, mask, grav)
     if not args.no_implicit and not batch:
         out["implicit_euler"] = implicit_report(P, rest, tets, mask, material, grav)

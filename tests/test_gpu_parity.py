@@ -35,8 +35,8 @@ def oracle_pre(rest, tets, mask, grav=(0.0, -9.8, 0.0)):
 
 # ---------------------------------------------------------------------------------------------
 def test_setup_data_bit_exact():
-    for name in ["c0", "model_bar"]:
-        if name == "c0":
+    for name in ["bar4x4x16", "model_bar"]:
+        if name == "bar4x4x16":
             rest, tets, mask, _ = bar_case()
         else:
             rest, tets = M.read_tetgen(os.path.join(M.GOLDEN, "model_bar/bar.1"))
@@ -141,7 +141,7 @@ def test_positions_after_100_steps_c0():
     (ascending vs descending gather, both FP32), times 3, and never below 1e-4 L.  The
     single-pass force and Hessian tests and the FP64 tracking test pin the arithmetic itself."""
     rest, tets, mask, x0 = bar_case()
-    say("c0 100 steps REF_GPU fp32")
+    say("4x4x16 bar 100 steps REF_GPU fp32")
     xg, xo, gst, ost = run_pair(rest, tets, mask, x0, 100, P.default_params(), O.default_params(), 32)
     o = O.Solver(rest, tets, mask, params=O.default_params(gather_order=1), precision=32)
     o.set_state(x=x0.astype(np.float64))
@@ -163,14 +163,14 @@ def test_positions_ref_cpu_semantics():
     FP64, where GPU and oracle make the same decisions; FP32 is checked over 10 steps."""
     rest, tets, mask, x0 = bar_case()
     L = 16 * 0.05
-    say("c0 100 steps REF_CPU fp64")
+    say("4x4x16 bar 100 steps REF_CPU fp64")
     xg, xo, gst, ost = run_pair(rest, tets, mask, x0, 100, P.default_params(semantics=P.SEM_REF_CPU, precision=P.FP64),
                                 O.default_params(semantics=O.REF_CPU), 64)
     err = np.abs(xg - xo).max()
     say("fp64 REF_CPU err %.2e L" % (err / L), "rollbacks", sum(g.rollbacks for g in gst), sum(q.rollbacks for q in ost))
     assert err <= 1e-4 * L
     assert sum(g.rollbacks for g in gst) > 50
-    say("c0 10 steps REF_CPU fp32")
+    say("4x4x16 bar 10 steps REF_CPU fp32")
     xg, xo, _, _ = run_pair(rest, tets, mask, x0, 10, P.default_params(semantics=P.SEM_REF_CPU),
                             O.default_params(semantics=O.REF_CPU), 32)
     assert np.abs(xg - xo).max() <= 1e-4 * L
@@ -178,7 +178,7 @@ def test_positions_ref_cpu_semantics():
 
 def test_fp64_mode_tracks_fp64_oracle():
     rest, tets, mask, x0 = bar_case()
-    say("c0 fp64 30 steps")
+    say("4x4x16 bar fp64 30 steps")
     xg, xo, gst, ost = run_pair(rest, tets, mask, x0, 30, P.default_params(precision=P.FP64), O.default_params(), 64)
     err = np.abs(xg - xo).max()
     say("fp64 max err", err)
