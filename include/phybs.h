@@ -76,9 +76,10 @@ typedef struct pbs_params {
   double mixed_switch;  /* PBS_FP64 with graphs only (no reference counterpart): while |g|/|g_0| of
                            the last graph replay is above this, the tile pass forms F in FP64 and
                            evaluates the per-tet forces in FP32 (neo-Hookean: from H = F - I) with
-                           FP32 contributions; energies stay FP64; then FP64 to the end of the step.
-                           0 = never (default).  A value above relgrad_tol keeps the final passes
-                           and the convergence test in FP64. */
+                           FP32 contributions; energies stay FP64; then FP64 to the end of the step
+                           (also as soon as |g| <= 2e-4 |f|, the FP32 forces' rounding scale, or
+                           after a rollback).  Convergence is only declared on FP64 passes.
+                           0 = never (default). */
 } pbs_params;
 
 typedef struct pbs_step_stats {

@@ -210,6 +210,7 @@ template <typename R> static int alloc_impl(pbs_solver *s) {
   rc |= dalloc(s, &hslot, H);
   rc |= dalloc(s, &s->td.tpart, (size_t)P.ntiles);
   rc |= dalloc(s, &s->td.npart, 2 * (size_t)std::max(s->td.nblk, 1) + 2);
+  rc |= dalloc(s, &s->td.fpart, (size_t)std::max(s->td.nblk, 1));
   rc |= dalloc(s, &m.fown, 3 * N);
   rc |= dalloc(s, &m.hpart, 3 * H);
   rc |= dalloc(s, &m.fownh, 3 * N);
@@ -777,9 +778,12 @@ template <typename R> static int step_impl(pbs_solver *s, const float *ext, floa
     return true;
   };
   // mixed mode: FP32 per-tet arithmetic until the last replay's |g|/|g0| (largest over the active
-  // instances) is at or below mixed_switch, FP64 from then on.  A rollback during an FP32 replay
-  // means the FP32 forces no longer resolve the remaining gradient: the step goes on in FP64 from
-  // the restored iterate, with the step length the step started with.
+  // instances) is at or below mixed_switch, or some instance's |g| is within kMixedNoise of its
+  // internal-force norm |f| (the FP32 forces' rounding, ~5e-5 |f| measured, would then dominate
+  // g), FP64 from then on.  A rollback during an FP32 replay means the FP32 forces no longer
+  // resolve the remaining gradient: the step goes on in FP64 from the restored iterate, with the
+  // step length the step started with.
+  constexpr double kMixedNoise = 2e-4;
   const bool mixed = use_graph && relg && s->p.mixed_switch > 0 && sizeof(R) == 8;
   bool fp64_phase = !mixed;
   std::vector<float> alpha0;
@@ -797,12 +801,14 @@ template <typename R> static int step_impl(pbs_solver *s, const float *ext, floa
       if (!fp64_phase) {
         double worst = -1.0;
         bool known = true;
+        bool noisy = false;
         for (const Ctrl &q : ch)
           if (!q.done) {
             if (!(q.g0 > 0.0)) known = false;
             else worst = std::max(worst, q.g / q.g0);
+            if (q.g <= kMixedNoise * q.fnorm) noisy = true;
           }
-        if (launched > 0 && known && worst <= s->p.mixed_switch) fp64_phase = true;
+        if (launched > 0 && ((known && worst <= s->p.mixed_switch) || noisy)) fp64_phase = true;
       }
       s->dp.f32math = fp64_phase ? 0 : 1;
       int rc = graph_ready<R>(s, G, 0);

@@ -40,6 +40,7 @@ struct Ctrl {
   float alpha, rho, omega, rho_next, omega_next, ws_beta;  // ws_beta: warm start's chosen step
   double E, E_old, g0, g;
   double g_plain, g_learned;  // warm start: gradient norms of the two initial guesses
+  double fnorm;               // mixed mode: |f| (internal forces on free nodes) of the last FP32 pass
 };
 
 // Tile partition on the device (see pbs_tiles.h for the host-side construction).
@@ -60,6 +61,7 @@ struct TileDev {
   const int *halo_slot;         // per halo entry: its partial slot
   double *tpart;                // per tile: elastic energy of its tets
   double *npart;                // per node block: node energy, |g|^2
+  double *fpart;                // per node block, mixed-mode FP32 passes only: |f|^2
 };
 
 // Values per corner contribution in the tile pass's LDS: 4 floats (one 16-byte access) or 3 doubles.

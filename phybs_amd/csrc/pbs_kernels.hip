@@ -38,6 +38,29 @@ __device__ __forceinline__ double block_sum(double v, double *lds) {
   return r;
 }
 
+// N block sums at once (one LDS exchange): the same order per value as block_sum; lds holds 16 N.
+template <int N> __device__ __forceinline__ void block_sums(double (&v)[N], double *lds) {
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[j] += __shfl_down(v[j], off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int j = 0; j < N; ++j) lds[16 * j + w] = v[j];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      double r = 0.0;
+      for (int i = 0; i < nw; ++i) r += lds[16 * j + i];
+      v[j] = r;
+    }
+  }
+  __syncthreads();
+}
+
 // Chebyshev schedule of the reference (CUDAInterface.cu:967-986; DescentOptimize.cpp:156-175).
 __device__ __forceinline__ void cheb_schedule(const DevParams &p, int k, float &rho, float &omega) {
   if (k == 0) {
@@ -445,7 +468,7 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
        const R *__restrict__ hpart, const float *__restrict__ fownh, const float *__restrict__ hparth,
        float *__restrict__ hd, const float *__restrict__ mass, const int *__restrict__ mask,
        const float *__restrict__ ext, const R *__restrict__ xt) {
-  __shared__ double lds_red[16];
+  __shared__ double lds_red[48];
   const int inst = td.blk_inst[blockIdx.x];
   ctrl += inst;
   if (ctrl->done) return;
@@ -456,7 +479,7 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
   R *__restrict__ xn = ring.b[ctrl->nxt];
   const int n = td.blk_start[blockIdx.x] + (int)threadIdx.x;
   const int N = p.nn;
-  double en = 0.0, g2 = 0.0;
+  double en = 0.0, g2 = 0.0, f2 = 0.0;
   if (n < td.inst_node_off[inst + 1]) {
     R f[3];
     float hv[3];
@@ -504,6 +527,7 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
         const R gr = dd * mh2 - f[d] - (R)p.ext_scale * fe;
         e += (R)0.5 * dd * dd * mh2 - fe * xc;
         g2 += (double)gr * (double)gr;
+        f2 += (double)f[d] * (double)f[d];
         if (!GRAD_ONLY) {
           R nx = xc - alpha * gr / ((R)hv[d] + mh2);
           if (cheb) {
@@ -516,11 +540,21 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
       if (!GRAD_ONLY) en += (double)e;
     }
   }
-  const double se = block_sum(en, lds_red);
-  const double sg = block_sum(g2, lds_red);
-  if (threadIdx.x == 0) {
-    td.npart[2 * blockIdx.x] = se;
-    td.npart[2 * blockIdx.x + 1] = sg;
+  if (!GRAD_ONLY && p.f32math) {  // mixed mode: also |f|^2, the scale of the FP32 force rounding
+    double v[3] = {en, g2, f2};
+    block_sums<3>(v, lds_red);
+    if (threadIdx.x == 0) {
+      td.npart[2 * blockIdx.x] = v[0];
+      td.npart[2 * blockIdx.x + 1] = v[1];
+      td.fpart[blockIdx.x] = v[2];
+    }
+  } else {
+    double v[2] = {en, g2};
+    block_sums<2>(v, lds_red);
+    if (threadIdx.x == 0) {
+      td.npart[2 * blockIdx.x] = v[0];
+      td.npart[2 * blockIdx.x + 1] = v[1];
+    }
   }
 }
 
@@ -528,7 +562,7 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
 // energies (check passes) and node-block partials (energy, |g|^2), then the loop control of the
 // reference.
 __global__ void __launch_bounds__(1024) k_control(DevParams p, Ctrl *__restrict__ ctrl, TileDev td) {
-  __shared__ double la[1024], lc[1024];
+  __shared__ double la[1024], lc[1024], lf[1024];
   const int inst = blockIdx.x;
   ctrl += inst;
   if (ctrl->done) return;
@@ -540,19 +574,27 @@ __global__ void __launch_bounds__(1024) k_control(DevParams p, Ctrl *__restrict_
     a += td.npart[2 * i];
     c += td.npart[2 * i + 1];
   }
+  const bool mix = p.f32math != 0;  // mixed mode: also |f|^2 of this pass, the FP32 force scale
+  double f = 0.0;
+  if (mix)
+    for (int i = td.inst_blk_off[inst] + tid; i < td.inst_blk_off[inst + 1]; i += 1024) f += td.fpart[i];
   la[tid] = a;
   lc[tid] = c;
+  lf[tid] = f;
   __syncthreads();
   for (int st = 512; st > 0; st >>= 1) {
     if (tid < st) {
       la[tid] += la[tid + st];
       lc[tid] += lc[tid + st];
+      if (mix) lf[tid] += lf[tid + st];
     }
     __syncthreads();
   }
+  const double E = la[0], g2 = lc[0], f2 = lf[0];
   if (tid == 0) {
     Ctrl C = *ctrl;
-    control_apply(p, C, la[0], lc[0]);
+    if (p.f32math) C.fnorm = sqrt(f2);
+    control_apply(p, C, E, g2);
     *ctrl = C;
   }
 }
