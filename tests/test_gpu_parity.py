@@ -281,6 +281,32 @@ def test_mixed_mode_converges_like_fp64_oracle(switch):
     assert err <= 1e-6
 
 
+def test_mixed_mode_tracks_fp64_on_a_strongly_twisted_bar():
+    """The 12x12x50 bar released from a 90 degree twist, gravity -z: its internal forces are large
+    against the step's gradient, so the FP32 passes reach their rounding floor (|g| ~ 5e-5 |f|)
+    before |g|/|g0| = 2e-4 and the step must go on in FP64 by the |f| test, without rollbacks: the
+    iteration counts then match the FP64 run's and the positions stay within the rel-grad mode's
+    own tolerance of it."""
+    rest, tets, mask, x0 = bar_case(12, 12, 50)
+    say("mixed mode on the twisted 12x12x50 bar")
+    out = []
+    for sw in (0.0, 2e-4):
+        gp = P.default_params(precision=P.FP64, relgrad_tol=1e-4, max_iterations=20000, mixed_switch=sw,
+                              grav=(0.0, 0.0, -9.8), profile=[(1, 0.96), (7, 0.991), (10, 0.9995)])
+        s = P.Solver(rest, tets, mask, gp)
+        s.set_positions(x0)
+        st = [s.step() for _ in range(12)]
+        out.append((s.positions(f64=True), st))
+        s.close()
+    (x64, s64), (xmx, smx) = out
+    say("iterations fp64", [q.iterations for q in s64], "mixed", [q.iterations for q in smx])
+    assert all(q.converged and q.rollbacks == 0 for q in smx)
+    assert abs(sum(q.iterations for q in smx) - sum(q.iterations for q in s64)) <= 0.02 * sum(q.iterations for q in s64)
+    err = np.abs(xmx - x64).max()
+    say("max |x_mixed - x_fp64|", err)
+    assert err <= 1e-6 * 2.5  # 1e-6 of the bar length (2.5 m)
+
+
 def test_mixed_mode_rejects_fp32_state():
     rest, tets, mask, _ = bar_case(1, 1, 2, twist=0.0, jitter=0.0)
     with pytest.raises(P.PhybsError):
