@@ -86,10 +86,17 @@ def cpu_baseline(rest, tets, mask, material, iters_per_step, grav, seconds=15.0)
     o.step()
     dt = time.perf_counter() - t0
     its = n / dt
+    # one thread (the reference's CPU descent is serial): 8 iterations, reported beside the value
+    p.iterations = 8
+    o1 = O.Solver(rest, tets, mask, params=p, precision=64, material=material, threads=1)
+    t0 = time.perf_counter()
+    o1.step()
+    its1 = 8 / (time.perf_counter() - t0)
     return {"value": its / max(iters_per_step, 1.0), "unit": "steps/s", "cores": cores, "kind": "port",
             "sample": f"{n} FP64 descent iterations of the same mesh in {dt:.1f} s "
                       f"({its:.2f} it/s), divided by the GPU's {iters_per_step:.0f} iterations/step",
-            "iterations_per_s": its}
+            "iterations_per_s": its, "single_thread_iterations_per_s": its1,
+            "single_thread_steps_per_s": its1 / max(iters_per_step, 1.0)}
 
 
 def cpu_baseline_batch(inst, iters, grav, sample=8):
