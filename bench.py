@@ -411,7 +411,8 @@ def main():
     iters = sum(st.iterations for st in stats)
     passes = sum(st.passes for st in stats)
     nt, nn = len(tets), len(rest)
-    alg = algorithmic_bytes(nt, nn, 8 if f64 else 4)
+    rsz = 8 if f64 else 4
+    alg = algorithmic_bytes(nt, nn, rsz)
     dom = max((k for k in ("k_tile", "k_tile_mixed", "k_node") if k in kt), key=lambda k: kt[k][0] * kt[k][1])
     avg_ms = kt[dom][0]
     achieved = alg[dom] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
@@ -438,6 +439,14 @@ def main():
                    "dt": 1.0 / 30.0, "tets": nt, "nodes": nn, "mixed_switch": params.mixed_switch},
         "iterations_per_step": iters / args.steps,
         "descent_iterations_per_s": iters / elapsed,
+        # whole descent pass against its compulsory bytes, 56 T + (12 r + 20) N with r bytes per
+        # position component: rest data and connectivity per tet; x, x~, x_last read, x_next
+        # written, Jacobi diagonal, mass and mask per node (the per-node forces k_tile hands to
+        # k_node are this design's staging, not compulsory)
+        "pass_roofline": {"bytes_per_pass": 56.0 * nt + (12.0 * rsz + 20.0) * nn,
+                          "us_per_pass": 1e6 * elapsed / max(passes, 1),
+                          "achieved_gbs": (56.0 * nt + (12.0 * rsz + 20.0) * nn) * max(passes, 1) / elapsed / 1e9,
+                          "frac": (56.0 * nt + (12.0 * rsz + 20.0) * nn) * max(passes, 1) / elapsed / 1e9 / HBM_PEAK_GBS},
         "rollbacks": sum(st.rollbacks for st in stats),
         "converged_steps": sum(st.converged for st in stats),
         "rel_grad_final": [st.g_norm / st.g0_norm if st.g0_norm > 0 else None for st in stats],
