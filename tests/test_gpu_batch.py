@@ -64,6 +64,36 @@ def test_batch_equals_separate_solvers(mode):
         off += len(x)
 
 
+def test_ragged_batch_with_a_single_tet_instance_equals_separate_solvers():
+    """Instances of 1, 6 and 96 tets side by side (tiles of one instance never mix with
+    another's): each instance's trajectory is bitwise that of its own solver."""
+    one = (np.array([[0, 0, 0], [0.05, 0, 0], [0, 0.05, 0], [0, 0, 0.05]], np.float32), np.array([[0, 1, 2, 3]], np.int32))
+    inst = [one, M.bar(1, 1, 1, 0.05), M.bar(2, 2, 4, 0.05)]
+    masks = [M.clamp_x0(r) for r, _ in inst]
+    x0s = [r + np.float32(0.002) * np.sin(np.arange(r.size, dtype=np.float32)).reshape(r.shape) * (1 - m[:, None])
+           for (r, _), m in zip(inst, masks)]
+    kw = dict(tile_tets=64, grav=(0.0, 0.0, -9.8))
+    say("ragged batch", [len(t) for _, t in inst])
+    sep = []
+    for (r, t), m, x0 in zip(inst, masks, x0s):
+        s = P.Solver(r, t, m, P.default_params(**kw))
+        s.set_positions(x0)
+        for _ in range(3):
+            s.step()
+        sep.append(s.positions())
+        s.close()
+    b = P.Solver.batch([(r, t, m) for (r, t), m in zip(inst, masks)], [1e5] * 3, [0.45] * 3, P.default_params(**kw))
+    b.set_positions(np.concatenate(x0s))
+    for _ in range(3):
+        b.step()
+    xb = b.positions()
+    b.close()
+    off = 0
+    for i, x in enumerate(sep):
+        assert np.array_equal(xb[off:off + len(x)].view(np.uint32), x.view(np.uint32)), i
+        off += len(x)
+
+
 def test_batch_graph_replay_matches_eager():
     inst = instances(seed=3, k=4)
     res = []

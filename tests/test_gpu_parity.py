@@ -398,3 +398,46 @@ def test_inverted_rest_tet_is_rejected():
     with pytest.raises(P.PhybsError) as e:
         P.Solver(rest, bad, mask, P.default_params())
     assert e.value.code == P.ERR_MESH
+
+
+def single_tet():
+    rest = np.array([[0, 0, 0], [0.05, 0, 0], [0, 0.05, 0], [0, 0, 0.05]], np.float32)
+    tets = np.array([[0, 1, 2, 3]], np.int32)
+    return rest, tets, M.clamp_x0(rest)  # nodes 0, 2, 3 lie on x = 0: one free node
+
+
+def test_single_tet_tracks_fp64_oracle():
+    """The smallest mesh: one tet, one free node, pulled by gravity from a displaced start; FP64
+    against the FP64 oracle over 20 steps.  (Once the node rests, successive energies differ only
+    by rounding, so the line search's accept/rollback decisions, and with them the pass counts,
+    may differ from the oracle's; the positions may not.)"""
+    rest, tets, mask = single_tet()
+    x0 = rest.copy()
+    x0[1] += (0.004, -0.003, 0.002)
+    say("single tet fp64 20 steps")
+    xg, xo, gst, ost = run_pair(rest, tets, mask, x0, 20, P.default_params(precision=P.FP64),
+                                O.default_params(), 64)
+    err = np.abs(xg - xo).max()
+    say("single tet max err", err, "moved", np.abs(xo - x0).max())
+    assert np.abs(xo - x0).max() > 1e-4
+    assert err <= 1e-10
+    assert [g.passes for g in gst[:5]] == [q.passes for q in ost[:5]]  # while the node still moves
+
+
+@pytest.mark.parametrize("relgrad", [False, True])
+def test_all_nodes_fixed_leaves_the_mesh_in_place(relgrad):
+    """Every node clamped: nothing moves, the step ends (rel-grad: at once, |g0| = 0)."""
+    rest, tets, _, x0 = bar_case(2, 2, 3)
+    mask = np.ones(len(rest), np.int32)
+    kw = dict(precision=P.FP64, relgrad_tol=1e-4, max_iterations=500) if relgrad else {}
+    say("all nodes fixed, relgrad", relgrad)
+    s = P.Solver(rest, tets, mask, P.default_params(**kw))
+    s.set_positions(x0)
+    for _ in range(3):
+        st = s.step()
+        assert np.isfinite(st.energy)
+    assert np.array_equal(s.positions(), x0)
+    if relgrad:
+        assert st.converged == 1 and st.iterations == 0
+    s.close()
+
