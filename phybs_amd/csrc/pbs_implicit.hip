@@ -522,6 +522,8 @@ __global__ void k_pd_begin(DevParams p, IeDev d, double *__restrict__ srhs, cons
 // of the singular values sorted in decreasing order).  With svd3's rotations U, V and the sign of
 // det F on the last singular value this is U V^T for det F >= 0 and U diag(-1, 1, -1) V^T otherwise.
 template <typename R> __device__ __forceinline__ void pd_rotation(const R *F, R *Rot) {
+  // det F > 0: U V^T is the rotation of the polar decomposition (scaled Newton iteration)
+  if (polar_rotation(F, Rot)) return;
   R U[9], S[3], V[9];
   svd3(F, U, S, V);
   const R s0 = S[2] < (R)0 ? (R)-1 : (R)1;
