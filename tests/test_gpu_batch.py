@@ -94,6 +94,39 @@ def test_ragged_batch_with_a_single_tet_instance_equals_separate_solvers():
         off += len(x)
 
 
+def test_batch_mixed_mode_tracks_fp64():
+    """Mixed mode on a batch: the switch to FP64 is taken for all instances together (the largest
+    |g|/|g0| among the active ones decides); every instance still converges to 1e-4 on FP64 passes.
+    Two iterates that both meet the tolerance differ by the tolerance's own slack, which is
+    largest on the softest bars (E down to 1e4 Pa): the end positions of every instance agree with
+    the FP64 batch's to 1e-3 of that instance's motion over the three steps."""
+    inst = instances(seed=5, k=4)
+    out = []
+    for sw in (0.0, 2e-4):
+        kw = dict(tile_tets=64, grav=(0.0, 0.0, -9.8), precision=P.FP64, relgrad_tol=1e-4, max_iterations=5000,
+                  mixed_switch=sw)
+        b = P.Solver.batch([(r, t, m) for r, t, m, _, _, _ in inst], [i[4] for i in inst], [i[5] for i in inst],
+                           P.default_params(**kw))
+        b.set_positions(np.concatenate([i[3] for i in inst]))
+        conv = []
+        for _ in range(3):
+            b.step()
+            conv.append([b.instance_stats(i) for i in range(len(inst))])
+        out.append(b.positions(f64=True))
+        b.close()
+        for row in conv:
+            for q in row:
+                assert q.converged == 1 and q.g_norm <= 1e-4 * q.g0_norm
+    off = 0
+    for r, t, m, x0, E, nu in inst:
+        sl = slice(off, off + len(r))
+        motion = np.abs(out[0][sl] - x0).max()
+        diff = np.abs(out[0][sl] - out[1][sl]).max()
+        say("instance E", E, "motion", motion, "mixed vs fp64", diff)
+        assert diff <= 1e-3 * motion
+        off += len(r)
+
+
 def test_batch_graph_replay_matches_eager():
     inst = instances(seed=3, k=4)
     res = []
