@@ -123,7 +123,7 @@ def cpu_baseline_batch(inst, iters, grav, sample=8):
 
 # the learned correction is smoothed by 100 averaging sweeps and tried at 2, 1.5 and 1 times its
 # length (the best of the three, if better than the plain guess, starts the descent)
-WARM_OPTIONS = dict(sweeps=140, betas=(1.5, 1.0))
+WARM_OPTIONS = dict(sweeps=140, betas=(1.0,))
 
 
 def impulse(rest, rng, k):
