@@ -11,10 +11,14 @@
 //                           getExternalForces(), getTets(), getConstraintIDs(), getE(), getNu(),
 //                           getNodesNum(), getTetsNum(), resetExternalForce().  It runs the GPU
 //                           solver with the CPU path's semantics (PBS_SEM_REF_CPU).
+//  * phybs::TimeIntegrationT<Vec, Mat>
+//                           the time-step holder the application reads the step from
+//                           (FVM/TimeIntegration.h:13-31; fvm.cpp:19-20, 55, 165, 187).
 //
 // Define PHYBS_REFERENCE_NAMES before including this header (after the reference's TetMesh.h and
-// IsotropicNeohookeanMaterial.h) to get the reference's global names CUDAInterface and
-// DescentOptimize, so that the application's call sites compile unchanged.
+// IsotropicNeohookeanMaterial.h) to get the reference's global names CUDAInterface,
+// DescentOptimize and (with Eigen included) TimeIntegration, so that the application's call
+// sites compile unchanged.
 #ifndef PHYBS_HPP
 #define PHYBS_HPP
 
@@ -182,11 +186,62 @@ class DescentOptimizeT {
   std::vector<float> v_last_;
 };
 
+// The reference's time-integration object as the application uses it: the time step, with the
+// defaults of the reference's three constructors (1e-6, 2e-3 and 0.03 s; TimeIntegration.cpp:6-30),
+// explicit Euler (simuExplicit, TimeIntegration.cpp:57-67) and the position / velocity getters.
+// Vec and Mat are the caller's vector and 3 x N column-major matrix types (Eigen::VectorXf,
+// Eigen::MatrixXf): data() and Mat::Zero(rows, cols) are all this class uses.  BackEuler with a
+// stiffness matrix the caller assembled on the CPU (TimeIntegration.cpp:69-118, factorised with
+// MKL Pardiso) is not provided and throws PBS_ERR_STATE: the GPU backward-Euler step is
+// CUDAInterface::computeInnerforces / computeGlobalStiffnessMatrix / doBackEuler.
+template <class Vec, class Mat>
+class TimeIntegrationT {
+ public:
+  explicit TimeIntegrationT(int n_nodes) : t_(1e-6f), n_(n_nodes) { zero(); }
+  TimeIntegrationT(int n_nodes, const Vec &) : t_(2e-3f), n_(n_nodes) { zero(); }
+  TimeIntegrationT(int n_nodes, const Vec &, const std::vector<int> &, const Mat &) : t_(0.03f), n_(n_nodes) {
+    zero();
+  }
+  // v' = v + t f / m, x' = x + t v' per node (pos, vel, force: 3 x N; mass: N)
+  void simuExplicit(const Mat &pos, const Mat &vel, const Mat &force, const Vec &mass) {
+    const float *p = pos.data(), *v = vel.data(), *f = force.data(), *m = mass.data();
+    float *xp = positions_.data(), *vp = velocities_.data();
+    for (int i = 0; i < n_; ++i)
+      for (int d = 0; d < 3; ++d) {
+        const size_t k = 3 * (size_t)i + d;
+        vp[k] = v[k] + t_ * f[k] / m[i];
+        xp[k] = p[k] + vp[k] * t_;
+      }
+  }
+  template <class Sparse>
+  void BackEuler(Mat &, Mat &, Mat &, Sparse &) {
+    throw Error(PBS_ERR_STATE,
+                "phybs: TimeIntegration::BackEuler with a CPU-assembled K is not provided; "
+                "use CUDAInterface::computeInnerforces/computeGlobalStiffnessMatrix/doBackEuler");
+  }
+  void setTimeStep(float t) { t_ = t; }
+  Mat getPositions() { return positions_; }
+  Mat getVelocities() { return velocities_; }
+  float &getTimeStep() { return t_; }
+
+ private:
+  void zero() {
+    positions_ = Mat::Zero(3, n_);
+    velocities_ = Mat::Zero(3, n_);
+  }
+  float t_;
+  int n_;
+  Mat positions_, velocities_;
+};
+
 }  // namespace phybs
 
 #ifdef PHYBS_REFERENCE_NAMES
 using phybs::CUDAInterface;
 typedef phybs::DescentOptimizeT<TetMesh, IsotropicNeohookeanMaterial> DescentOptimize;
+#ifdef EIGEN_WORLD_VERSION
+typedef phybs::TimeIntegrationT<Eigen::VectorXf, Eigen::MatrixXf> TimeIntegration;
+#endif
 #endif
 
 #endif  // PHYBS_HPP
