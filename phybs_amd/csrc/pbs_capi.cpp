@@ -600,6 +600,7 @@ extern "C" int pbs_set_params(pbs_solver *s, const pbs_params *p) {
   } else {
     Launch<float>::setup_nodes(s->m32, s->dp, 256, s->st);
   }
+  implicit_params_changed(s->implicit);  // the projective-dynamics matrix is rebuilt
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->st));
   return PBS_OK;

@@ -15,6 +15,10 @@
 
 namespace pbs {
 
+void implicit_params_changed(ImplicitState *S) {
+  if (S) S->pd_built = false;
+}
+
 void implicit_release(ImplicitState *S) {
   if (!S) return;
   for (CgGraphs *G : {&S->cg_ie, &S->cg_pd}) {
@@ -313,8 +317,12 @@ template <typename R> int pd_get(pbs_solver *s, ImplicitState **out) {
     const TilePlan &P = s->plan;
     for (size_t e = 0; e < P.inc.size(); ++e) pos[P.inc[e]] = (int)e;
     HIPCHK(hipMemcpy(S->pd_inc_pos, pos.data(), sizeof(int) * pos.size(), hipMemcpyHostToDevice));
-    S->pd = S->dev;  // same layout, vectors and control block
+    double *dinv = nullptr;
+    rc = ialloc(S, &dinv, 3 * (size_t)s->nn);
+    if (rc) return rc;
+    S->pd = S->dev;  // same layout, vectors and control block; its own matrix and Jacobi diagonal
     S->pd.A = A;
+    S->pd.dinv = dinv;
     S->pd.scalar = 1;
   }
   *out = S;
@@ -339,7 +347,12 @@ int pd_step_impl(pbs_solver *s, const pbs_pd_params *pp, const float *ext, float
   DevMesh<R> &m = mesh<R>(s);
   IeDev &d = S->pd;
   HIPCHK(hipEventRecord(s->ev0, s->st));
-  IeLaunch<R>::pd_matrix(m, s->dp, pp->stiffness, d, s->st);
+  if (!S->pd_built || S->pd_stiffness != pp->stiffness || S->pd_h != (double)s->dp.h) {
+    IeLaunch<R>::pd_matrix(m, s->dp, pp->stiffness, d, s->st);
+    S->pd_built = true;
+    S->pd_stiffness = pp->stiffness;
+    S->pd_h = (double)s->dp.h;
+  }
   IeLaunch<R>::pd_begin(m, s->dp, d, S->pd_s, ext ? 1 : 0, s->st);
   HIPCHK(hipGetLastError());
   long its = 0;

@@ -73,6 +73,11 @@ struct ImplicitState {
   IeDev dev{};                                      // backward Euler: 3x3 blocks
   IeDev pd{};                                       // projective dynamics: same layout and vectors, scalar values
   double *pd_s = nullptr;                           // projective dynamics: momentum term (3N)
+  // The PD matrix depends only on the rest shape, the masses, the stiffness and the time step: it is
+  // built once per (stiffness, h) with its own Jacobi diagonal and reused by later steps.
+  bool pd_built = false;
+  float pd_stiffness = 0.0f;
+  double pd_h = 0.0;
   int *pd_inc_pos = nullptr;                        // projective dynamics: incidence slot of each (tet, corner)
   std::vector<int> slice_off_h, row_len_h, bcol_h;  // host copies (stiffness export)
   CgGraphs cg_ie, cg_pd;
@@ -81,5 +86,6 @@ struct ImplicitState {
 };
 
 void implicit_release(ImplicitState *S);
+void implicit_params_changed(ImplicitState *S);
 
 }  // namespace pbs

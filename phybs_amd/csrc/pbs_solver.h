@@ -17,6 +17,7 @@ struct LearnState;  // pbs_learn.h
 void learn_release(LearnState *L);
 struct ImplicitState;  // pbs_implicit.h
 void implicit_release(ImplicitState *S);
+void implicit_params_changed(ImplicitState *S);  // masses or time step may have changed
 }  // namespace pbs
 
 using namespace pbs;

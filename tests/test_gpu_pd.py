@@ -111,3 +111,35 @@ def test_pd_step_is_deterministic_and_keeps_the_descent_usable():
         assert np.isfinite(st.energy) and np.isfinite(s.positions()).all()
         s.close()
     assert np.array_equal(res[0].view(np.uint32), res[1].view(np.uint32))
+
+
+def test_cached_matrix_follows_stiffness_and_parameters():
+    """The PD matrix is built once per (stiffness, time step) and rebuilt when either changes or
+    the parameters (masses) change: a step after such a change equals, bit for bit, the same step
+    of a fresh solver put into the same state (FP32 state, so the state copies exactly)."""
+    rest, tets, mask, x0 = case(seed=3)
+    say("pd matrix cache")
+
+    def fresh(state, **kw):
+        s = P.Solver(rest, tets, mask, P.default_params(**kw))
+        s.set_positions(state[0])
+        s.set_velocities(state[1], state[2])
+        return s
+
+    a = P.Solver(rest, tets, mask, P.default_params())
+    a.set_positions(x0)
+    a.pd_step(params=P.pd_params(stiffness=1e6))
+    a.pd_step(params=P.pd_params(stiffness=1e6))  # reuses the matrix
+    st = (a.positions(), *a.velocities())
+    a.pd_step(params=P.pd_params(stiffness=3e6))  # rebuilds it
+    b = fresh(st)
+    b.pd_step(params=P.pd_params(stiffness=3e6))
+    assert np.array_equal(a.positions().view(np.uint32), b.positions().view(np.uint32))
+    st = (a.positions(), *a.velocities())
+    a.set_params(P.default_params(density=2500.0))  # new masses
+    a.pd_step(params=P.pd_params(stiffness=3e6))
+    c = fresh(st, density=2500.0)
+    c.pd_step(params=P.pd_params(stiffness=3e6))
+    assert np.array_equal(a.positions().view(np.uint32), c.positions().view(np.uint32))
+    for s in (a, b, c):
+        s.close()
