@@ -180,14 +180,20 @@ def timed_steps(P, rest, tets, mask, x0, params, warmup, steps, model=None):
             "converged_steps": int(sum(q.converged for q in st)), "rollbacks": int(sum(q.rollbacks for q in st))}
 
 
-def warm_start_report(P, args, params, rest, tets, mask, x0, dims, plain_sps, plain_its):
+def warm_start_report(P, args, params, rest, tets, mask, x0, dims, plain_sps, plain_its, other=None):
     """Learned warm start vs plain: on the benchmark bar (trained on another trajectory of the same
-    bar) and on the 12x12x50 bar (43,200 tets)."""
+    bar) and on the 12x12x50 bar (43,200 tets).  other = (params, plain result) of the other
+    arithmetic (mixed / FP64 throughout): the same trained network there too."""
     out = {}
     model, info = train_warm_start(P, dims, params, args.train_steps, 7, 30, 14_000_000)
     lw = timed_steps(P, rest, tets, mask, x0, params, args.warmup, args.steps, model)
     out["bench_bar"] = {"training": info, "plain_steps_per_s": plain_sps, "plain_iterations_per_step": plain_its,
                         "learned": lw, "speedup": lw["steps_per_s"] / plain_sps}
+    if other is not None:
+        op, oplain = other
+        lo = timed_steps(P, rest, tets, mask, x0, op, args.warmup, args.steps, model)
+        out["bench_bar_other_arithmetic"] = {"mixed_switch": op.mixed_switch, "plain_steps_per_s": oplain["steps_per_s"],
+                                             "learned": lo, "speedup": lo["steps_per_s"] / oplain["steps_per_s"]}
     d1 = (12, 12, 50)
     r1, t1 = P.mesh_bar(*d1, 0.05)
     m1 = (r1[:, 0] == 0).astype(np.int32)
@@ -464,12 +470,14 @@ def main():
         # the same steps with the other arithmetic: mixed mode (FP32 per-tet arithmetic down to
         # |g|/|g0| = 2e-4) when the main run was FP64 throughout, and the reverse
         sw = 0.0 if params.mixed_switch > 0 else 2e-4
-        r = timed_steps(P, rest, tets, mask, x0, P.default_params(**pkw, mixed_switch=sw), args.warmup, args.steps)
+        other_params = P.default_params(**pkw, mixed_switch=sw)
+        r = timed_steps(P, rest, tets, mask, x0, other_params, args.warmup, args.steps)
         r["mixed_switch"] = sw
         out["other_arithmetic"] = r
     if relg and not args.no_learned and not batch:
+        other = (other_params, out["other_arithmetic"]) if f64 else None
         out["learned_warm_start"] = warm_start_report(P, args, params, rest, tets, mask, x0, (nx, ny, nz),
-                                                      steps_per_s, iters / args.steps)
+                                                      steps_per_s, iters / args.steps, other)
     if not batch and args.workload == "bar64x64x256":
         out["fixed_64_iterations_f32"] = fixed_iteration_report(P, rest, tets, mask, grav)
     if not args.no_implicit and not batch:
