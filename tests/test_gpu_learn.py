@@ -261,6 +261,18 @@ def test_trained_warm_start_gain():
     s.close()
     sp, stp = run(11, 30)
     sl, stl = run(11, 30, mdl)
+    # the same network under the mixed mode (FP32 forces down to |g|/|g0| = 2e-4): every step
+    # still converges on FP64 passes, with the warm-start decision of the FP64 run's first step
+    prm_fp64 = prm
+    prm = P.default_params(precision=P.FP64, relgrad_tol=1e-4, max_iterations=20000, dt=1 / 30.0,
+                           grav=(0.0, 0.0, -9.8), profile=[(1, 0.96), (7, 0.991), (10, 0.9995)], mixed_switch=2e-4)
+    sm, stm = run(11, 10, mdl)
+    prm = prm_fp64
+    assert all(q.converged and q.g_norm <= 1e-4 * q.g0_norm for q in stm)
+    assert stm[0].used_learned == stl[0].used_learned
+    say("mixed + warm start: iterations/step %.1f (FP64: %.1f over the same 10 steps)"
+        % (np.mean([q.iterations for q in stm]), np.mean([q.iterations for q in stl[:10]])))
+    sm.close()
     ip = np.mean([q.iterations for q in stp])
     il = np.mean([q.iterations for q in stl])
     used = np.mean([q.used_learned for q in stl])
