@@ -24,6 +24,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (each scaled by the
+# same counter's reading for a plain elementwise kernel of known size) on the 64x64x256 bar:
+# (kernel, bytes per position component) -> bytes.  Commands and raw figures:
+# profiles/k_tile_k_node_bar64x64x256_relgrad_f64.md.
+MEASURED_TRAFFIC = {("k_tile", 8): 476.6e6, ("k_tile", 4): 427.3e6, ("k_node", 8): 202.3e6, ("k_node", 4): 118.5e6}
+
 WORKLOADS = {
     # name: (nx, ny, nz, material, twist_deg, jitter, fixed iterations)
     "bar4x4x16": (4, 4, 16, 0, 0.0, 0.0, 96),
@@ -458,7 +464,9 @@ def main():
         "rel_grad_final": [st.g_norm / st.g0_norm if st.g0_norm > 0 else None for st in stats],
         "kernel_avg_ms": {k: v[0] for k, v in kt.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": MEASURED_TRAFFIC.get((dom, rsz)) if args.workload == "bar64x64x256" else None,
+                     "kernel": dom,
                      "algorithmic_bytes_per_launch": alg[dom]},
         "learned_warm_start": None,
     }
