@@ -493,3 +493,32 @@ def test_full_size_bar_forces_match_oracle():
     say("per-tet force rel err max", err.max(), "energy", eg, eo)
     assert err.max() <= 1e-5
     assert abs(eg - eo) <= 1e-6 * abs(eo)
+
+
+def test_full_size_relgrad_step_judged_by_the_oracle():
+    """One FP64 rel-grad 1e-4 step of the benchmark's 64x64x256 bar from rest: the oracle, given
+    the same step start, evaluates the incremental-potential gradient at the plain guess and at the
+    GPU's result; its norms agree with the solver's |g0| and |g| to 1e-6 and the GPU's result meets
+    rel-grad 1e-4 by the oracle's own measure."""
+    nx, ny, nz = 64, 64, 256
+    rest, tets = P.mesh_bar(nx, ny, nz, 0.05)
+    mask = (rest[:, 0] == 0).astype(np.int32)
+    grav, prof = (0.0, 0.0, -9.8), [(1, 0.96), (7, 0.991), (10, 0.9995)]
+    say("64x64x256 bar: one rel-grad step judged by the oracle")
+    s = P.Solver(rest, tets, mask, P.default_params(precision=P.FP64, relgrad_tol=1e-4, max_iterations=40000,
+                                                   dt=1.0 / 30.0, grav=grav, profile=prof))
+    st, _ = s.step(want_positions=True)
+    xg = s.positions(f64=True)
+    s.close()
+    assert st.converged == 1
+    threads = min(16, len(os.sched_getaffinity(0)))
+    o = O.Solver(rest, tets, mask, params=O.default_params(iterations=1, dt=1.0 / 30.0, grav=grav, profile=prof),
+                 precision=64, threads=threads)
+    o.set_state(x=rest.astype(np.float64))
+    o.step()  # sets the step start (x~ = x + h v) the gradients are taken against
+    g0 = o.gradient_norm(rest.astype(np.float64))
+    g = o.gradient_norm(xg)
+    say("oracle |g0|", g0, "gpu", st.g0_norm, "oracle |g|", g, "gpu", st.g_norm, "iterations", st.iterations)
+    assert abs(g0 - st.g0_norm) <= 1e-6 * g0
+    assert abs(g - st.g_norm) <= 1e-6 * g0
+    assert g <= 1e-4 * g0
