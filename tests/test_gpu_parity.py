@@ -473,21 +473,23 @@ def test_twisted_12x12x50_bar_forces_and_fp64_steps():
     s.close()
 
 
-def test_full_size_bar_forces_match_oracle():
-    """The 64x64x256 bar of the benchmark (6,291,456 tets) in a seeded twisted state: every
-    per-tet force of the FP64 path within 1e-5 of the oracle's, and the total elastic energy (the
-    per-tet values are returned in FP32) within 1e-6 relative."""
+@pytest.mark.parametrize("material", [P.MAT_NEOHOOKEAN, P.MAT_STRETCH])
+def test_full_size_bar_forces_match_oracle(material):
+    """The 64x64x256 bar of the benchmark (6,291,456 tets) in a seeded twisted state, both
+    materials (the stretch material's forces via the polar rotation, the oracle's via the SVD):
+    every per-tet force of the FP64 path within 1e-5 of the oracle's, and the total elastic energy
+    (the per-tet values are returned in FP32) within 1e-6 relative."""
     nx, ny, nz = 64, 64, 256
     rest, tets = P.mesh_bar(nx, ny, nz, 0.05)
     mask = (rest[:, 0] == 0).astype(np.int32)
     x = P.mesh_twist(rest, nx, ny, nz, 0.05, 30.0, 0.005, 3)
-    say("64x64x256 bar forces")
-    s = P.Solver(rest, tets, mask, P.default_params(precision=P.FP64))
+    say("64x64x256 bar forces, material", material)
+    s = P.Solver(rest, tets, mask, P.default_params(precision=P.FP64, material=material))
     tf, _, te = s.compute_forces(x)
     s.close()
     mu, lam = O.lame(1e5, 0.45)
     pre = O.precompute(rest, tets, mask, 1000.0, (0.0, -9.8, 0.0), 1e8)
-    rf, re_ = O.tet_forces(x.astype(np.float64), tets, pre, mu, lam, O.MAT_NEOHOOKEAN, O.CLAMP_GPU, 64)
+    rf, re_ = O.tet_forces(x.astype(np.float64), tets, pre, mu, lam, material, O.CLAMP_GPU, 64)
     err = per_tet_rel_err(tf, rf)
     eg, eo = te.astype(np.float64).sum(), re_.astype(np.float64).sum()
     say("per-tet force rel err max", err.max(), "energy", eg, eo)
