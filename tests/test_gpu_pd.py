@@ -143,3 +143,25 @@ def test_cached_matrix_follows_stiffness_and_parameters():
     assert np.array_equal(a.positions().view(np.uint32), c.positions().view(np.uint32))
     for s in (a, b, c):
         s.close()
+
+
+def test_full_size_local_step_matches_oracle():
+    """The local step on the 64x64x256 bar (6,291,456 tets) in a seeded twisted state with every
+    50th tet inverted (the rotation's sign convention, taken by the SVD branch; the other tets take
+    the polar iteration): FP64 within 1e-9 relative of the oracle's SVD-based projection."""
+    nx, ny, nz = 64, 64, 256
+    rest, tets = P.mesh_bar(nx, ny, nz, 0.05)
+    mask = (rest[:, 0] == 0).astype(np.int32)
+    x = P.mesh_twist(rest, nx, ny, nz, 0.05, 30.0, 0.005, 5).astype(np.float64)
+    for t in tets[::50]:  # reflect the first node through the opposite face
+        x[t[0]] = 2 * x[t[1:]].mean(axis=0) - x[t[0]]
+    x = x.astype(np.float32)
+    say("pd local step, 64x64x256 bar")
+    s = P.Solver(rest, tets, mask, P.default_params(precision=P.FP64))
+    got = s.pd_project(x, 1e6)
+    s.close()
+    pre = O.precompute(rest, tets, mask)
+    ref = O.pd_project(x.astype(np.float64), tets, pre, 1e6)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    say("max rel err", err)
+    assert err <= 1e-9
