@@ -84,6 +84,8 @@ int implicit_get(pbs_solver *s, ImplicitState **out) {
     S->slice_off_h[sl + 1] = S->slice_off_h[sl] + w;
   }
   S->slots = 64L * S->slice_off_h[S->nslice];
+  for (int sl = 0; sl < S->nslice; ++sl)
+    S->max_width = std::max(S->max_width, S->slice_off_h[sl + 1] - S->slice_off_h[sl]);
   S->bcol_h.assign(S->slots, 0);
   std::vector<int> diag(N);
   for (int n = 0; n < N; ++n) {
@@ -271,7 +273,7 @@ int implicit_step_impl(pbs_solver *s, const pbs_implicit_params *ip, const float
   HIPCHK(hipEventRecord(s->ev0, s->st));
   hipEvent_t *ea = timed ? &S->ev[4 * kGraphIters] : nullptr;  // events around the assembly
   if (ea) HIPCHK(hipEventRecord(ea[0], s->st));
-  IeLaunch<R>::assemble(m, s->td, s->dp, d, a, s->st);
+  IeLaunch<R>::assemble(m, s->td, s->dp, d, a, S->max_width, s->st);
   HIPCHK(hipGetLastError());
   if (ea) HIPCHK(hipEventRecord(ea[1], s->st));
   IeCtrl c{};
@@ -429,7 +431,7 @@ template <typename R> int stiffness_impl(pbs_solver *s, int64_t nnz, double *csr
   if (nnz != total) return set_error(PBS_ERR_ARG, "nnz does not match the stiffness pattern (" + std::to_string(total) + ")");
   IeAsm a{};
   a.raw = 1;
-  IeLaunch<R>::assemble(mesh<R>(s), s->td, s->dp, S->dev, a, s->st);
+  IeLaunch<R>::assemble(mesh<R>(s), s->td, s->dp, S->dev, a, S->max_width, s->st);
   HIPCHK(hipGetLastError());
   std::vector<R> A(9 * (size_t)S->slots);
   HIPCHK(hipMemcpyAsync(A.data(), S->dev.A, sizeof(R) * A.size(), hipMemcpyDeviceToHost, s->st));

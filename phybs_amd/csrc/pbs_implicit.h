@@ -42,8 +42,9 @@ struct IeAsm {
 };
 
 template <typename R> struct IeLaunch {
+  // max_width: the widest slice (slots per row)
   static void assemble(const DevMesh<R> &m, const TileDev &td, const DevParams &p, const IeDev &d, const IeAsm &a,
-                       hipStream_t st);
+                       int max_width, hipStream_t st);
   static void start(const IeDev &d, hipStream_t st);
   // one CG iteration = spmv, alpha, update, beta (+ the search-direction update), each a kernel
   static void spmv(const IeDev &d, hipStream_t st);
@@ -70,6 +71,7 @@ struct CgGraphs {
 struct ImplicitState {
   int nslice = 0;
   long slots = 0;
+  int max_width = 0;                                // slots per row of the widest slice
   IeDev dev{};                                      // backward Euler: 3x3 blocks
   IeDev pd{};                                       // projective dynamics: same layout and vectors, scalar values
   double *pd_s = nullptr;                           // projective dynamics: momentum term (3N)

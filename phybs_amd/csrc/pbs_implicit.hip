@@ -109,14 +109,18 @@ __device__ __forceinline__ int find_col(const int *__restrict__ bcol, int g0, in
 // derivative the reference evaluates through the SVD and Xu et al. 2015's 2x2 solves
 // (compute_dfdx, CUDAInterface.cu:101-219).  The elastic force of the corner comes from the same
 // per-tet routine as the descent solver's.
+// LACC: the row is accumulated in LDS (one 64-row slice per workgroup, each lane its own column
+// of slots, so no lane touches another's) and written to the matrix once; otherwise it is
+// accumulated in place in the matrix.
 // ---------------------------------------------------------------------------------------------
-template <typename R, int MAT>
+template <typename R, int MAT, bool LACC>
 __global__ void __launch_bounds__(kBlock)
     k_ie_assemble(DevParams p, IeAsm a, IeDev d, const int4 *__restrict__ tets, const float *__restrict__ dmi,
                   const float *__restrict__ vol, const int *__restrict__ inc_off, const int *__restrict__ inc,
                   const R *__restrict__ xpos, const R *__restrict__ vel, const float *__restrict__ rest,
                   const float *__restrict__ mass, const float *__restrict__ gforce, const float *__restrict__ ext,
                   const int *__restrict__ mask, const float *__restrict__ inst_mu, const float *__restrict__ inst_lam) {
+  extern __shared__ __align__(16) unsigned char ie_lds[];
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= d.n) return;
   R *__restrict__ A = (R *)d.A;
@@ -125,9 +129,11 @@ __global__ void __launch_bounds__(kBlock)
   const int g0 = d.slice_off[n >> 6];
   const int width = d.slice_off[(n >> 6) + 1] - g0;
   const int len = d.row_len[n];
+  R *__restrict__ acc = LACC ? (R *)ie_lds : A;
+  auto ix = [&](int j, int q) -> size_t { return LACC ? ((size_t)j * 9 + q) * 64 + lane : val_idx(g0 + j, q, lane); };
   for (int j = 0; j < width; ++j)
 #pragma unroll
-    for (int q = 0; q < 9; ++q) A[val_idx(g0 + j, q, lane)] = (R)0;
+    for (int q = 0; q < 9; ++q) acc[ix(j, q)] = (R)0;
   const R mu = (R)inst_mu[0], lam = (R)inst_lam[0];
   R f[3] = {(R)0, (R)0, (R)0};
   for (int e = inc_off[n]; e < inc_off[n + 1]; ++e) {
@@ -204,7 +210,7 @@ __global__ void __launch_bounds__(kBlock)
     const R mu2 = (MAT == 0) ? mu : (R)2 * mu;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      const int slot_g = g0 + find_col(d.bcol, g0, lane, len, qn[m]);
+      const int slot_j = find_col(d.bcol, g0, lane, len, qn[m]);
       const R cc = ci3[0] * c[m][0] + ci3[1] * c[m][1] + ci3[2] * c[m][2];
       R aj[3][3];
       if (MAT != 0) {
@@ -231,28 +237,38 @@ __global__ void __launch_bounds__(kBlock)
             for (int q = 0; q < 3; ++q) rot += ai[q][dd] * aj[q][ee] * aw[q];
             kv -= mu2 * rot;
           }
-          A[val_idx(slot_g, 3 * dd + ee, lane)] += V * kv;
+          acc[ix(slot_j, 3 * dd + ee)] += V * kv;
         }
     }
   }
-  if (a.raw) return;
+  auto flush = [&]() {  // LDS row -> matrix
+    if (LACC)
+      for (int j = 0; j < width; ++j)
+#pragma unroll
+        for (int q = 0; q < 9; ++q) A[val_idx(g0 + j, q, lane)] = acc[ix(j, q)];
+  };
+  if (a.raw) {
+    flush();
+    return;
+  }
   // system matrix (h + beta) h K + (1 + alpha h) M (doBackEuler, CUDAInterface.cu:845-851)
   const R s1 = (R)a.h + (R)a.beta, s2 = (R)a.h;
   for (int j = 0; j < len; ++j)
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
-      const size_t ix = val_idx(g0 + j, q, lane);
-      A[ix] = A[ix] * s1 * s2;
+      const size_t i = ix(j, q);
+      acc[i] = acc[i] * s1 * s2;
     }
   const R ms = (R)mass[n] * ((R)1 + (R)a.alpha * (R)a.h);
   const int ds = d.diag_slot[n];
-  const long dg = ds >> 6;
+  const int dj = (int)((ds >> 6) - g0);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const size_t ix = val_idx(dg, 4 * k, lane);
-    A[ix] += ms;
-    d.dinv[3 * (size_t)n + k] = 1.0 / (double)A[ix];
+    const size_t i = ix(dj, 4 * k);
+    acc[i] += ms;
+    d.dinv[3 * (size_t)n + k] = 1.0 / (double)acc[i];
   }
+  flush();
   // right-hand side h f + m v (setRHSofLinearSystem :418-442) with the forces of
   // addFixContraintsAndGravity (:327-343): fixed nodes a 1e8 spring to the rest position
   const R h = (R)a.h;
@@ -615,13 +631,28 @@ inline int nb_of(int n) { return (n + kBlock - 1) / kBlock; }
 
 template <typename R>
 void IeLaunch<R>::assemble(const DevMesh<R> &m, const TileDev &td, const DevParams &p, const IeDev &d,
-                           const IeAsm &a, hipStream_t st) {
+                           const IeAsm &a, int max_width, hipStream_t st) {
+  // rows accumulated in LDS when the widest slice fits in 64 KB per workgroup (two or more
+  // workgroups per CU), else in place
+  const size_t lds = (size_t)max_width * 9 * 64 * sizeof(R);
+  if (lds <= 65536) {
+    const dim3 grid((d.n + 63) / 64), blk(64);
+    if (p.material == 0)
+      hipLaunchKernelGGL((k_ie_assemble<R, 0, true>), grid, blk, lds, st, p, a, d, m.tets, m.dmi, m.vol, m.inc_off,
+                         m.inc, m.xpos, m.v, m.rest, m.mass, m.gforce, m.ext, m.mask, td.inst_mu, td.inst_lam);
+    else
+      hipLaunchKernelGGL((k_ie_assemble<R, 1, true>), grid, blk, lds, st, p, a, d, m.tets, m.dmi, m.vol, m.inc_off,
+                         m.inc, m.xpos, m.v, m.rest, m.mass, m.gforce, m.ext, m.mask, td.inst_mu, td.inst_lam);
+    return;
+  }
   if (p.material == 0)
-    hipLaunchKernelGGL((k_ie_assemble<R, 0>), dim3(nb_of(d.n)), dim3(kBlock), 0, st, p, a, d, m.tets, m.dmi, m.vol,
-                       m.inc_off, m.inc, m.xpos, m.v, m.rest, m.mass, m.gforce, m.ext, m.mask, td.inst_mu, td.inst_lam);
+    hipLaunchKernelGGL((k_ie_assemble<R, 0, false>), dim3(nb_of(d.n)), dim3(kBlock), 0, st, p, a, d, m.tets, m.dmi,
+                       m.vol, m.inc_off, m.inc, m.xpos, m.v, m.rest, m.mass, m.gforce, m.ext, m.mask, td.inst_mu,
+                       td.inst_lam);
   else
-    hipLaunchKernelGGL((k_ie_assemble<R, 1>), dim3(nb_of(d.n)), dim3(kBlock), 0, st, p, a, d, m.tets, m.dmi, m.vol,
-                       m.inc_off, m.inc, m.xpos, m.v, m.rest, m.mass, m.gforce, m.ext, m.mask, td.inst_mu, td.inst_lam);
+    hipLaunchKernelGGL((k_ie_assemble<R, 1, false>), dim3(nb_of(d.n)), dim3(kBlock), 0, st, p, a, d, m.tets, m.dmi,
+                       m.vol, m.inc_off, m.inc, m.xpos, m.v, m.rest, m.mass, m.gforce, m.ext, m.mask, td.inst_mu,
+                       td.inst_lam);
 }
 
 template <typename R> void IeLaunch<R>::start(const IeDev &d, hipStream_t st) {
