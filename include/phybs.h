@@ -172,6 +172,13 @@ int pbs_compute_forces(pbs_solver *s, const float *pos, float *tet_forces, float
 /* Diagonal of IsotropicMaterial::computeGlobalStiffnessMatrixFromPos (IsotropicMaterial.cpp:646-682),
  * i.e. of K = -df/dx: node_hd 3N, tet_hd 12T (either may be NULL). */
 int pbs_compute_hessian_diag(pbs_solver *s, const float *pos, float *node_hd, float *tet_hd);
+/* The per-node elastic forces and stiffness diagonal as the descent pass itself forms them: one
+ * launch of the tile kernel at pos (3N AoS) -- the per-tet half of computeForceKd
+ * (CUDAInterface.cu:495-568) -- then each node's owned partial plus its halo partials, in the
+ * order the node pass adds them.  node_forces 3N (elastic only, as pbs_compute_forces), node_hd
+ * 3N (NULL: no Hessian phase), energy: the pass's FP64 elastic energy (may be NULL).
+ * Single-mesh handles only. */
+int pbs_tile_forces(pbs_solver *s, const float *pos, double *node_forces, double *node_hd, double *energy);
 
 /* ---- measurement ---- */
 /* Average device time per launch (ms) of each kernel of the last pbs_step calls, measured with

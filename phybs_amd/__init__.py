@@ -119,6 +119,7 @@ SIGNATURES = {
     "pbs_do_descent_opt": (C.c_int, [_P, C.c_float, C.c_int, _P, _P]),
     "pbs_compute_forces": (C.c_int, [_P, _P, _P, _P, _P]),
     "pbs_compute_hessian_diag": (C.c_int, [_P, _P, _P, _P]),
+    "pbs_tile_forces": (C.c_int, [_P, _P, _P, _P, _P]),
     "pbs_kernel_times": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),
     "pbs_reset_kernel_times": (C.c_int, [_P]),
     "pbs_set_profiling": (C.c_int, [_P, C.c_int]),
@@ -453,6 +454,17 @@ class Solver:
         th = np.zeros((self.nt, 12), np.float32)
         _check(lib().pbs_compute_hessian_diag(self.h, _p(pos), _p(nh), _p(th)))
         return nh, th
+
+    def tile_forces(self, pos, hessian=False):
+        """Per-node elastic forces (and stiffness diagonal) as one descent pass forms them: the tile
+        kernel's owned and halo partials summed in the node pass's order.  Returns (forces (N, 3)
+        float64, hd (N, 3) float64 or None, elastic energy)."""
+        pos = _f32(pos)
+        nf = np.zeros((self.nn, 3), np.float64)
+        nh = np.zeros((self.nn, 3), np.float64) if hessian else None
+        e = C.c_double()
+        _check(lib().pbs_tile_forces(self.h, _p(pos), _p(nf), _p(nh), C.byref(e)))
+        return nf, nh, e.value
 
     def rest_data(self):
         dminv = np.zeros((self.nt, 9), np.float32)

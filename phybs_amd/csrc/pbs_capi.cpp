@@ -200,12 +200,13 @@ template <typename R> static int alloc_impl(pbs_solver *s) {
   const size_t H = std::max<long>(P.halo_total, 1);
   TileInfo *tiles;
   int *halo, *bnd_off, *hslot;
-  uint32_t *tlp;
-  unsigned short *toff;
+  uint32_t *tlp, *tgat;
+  unsigned short *tdst;
   rc |= dalloc(s, &tiles, (size_t)P.ntiles);
   rc |= dalloc(s, &halo, H);
   rc |= dalloc(s, &tlp, 4 * T);
-  rc |= dalloc(s, &toff, P.tile_off.size());
+  rc |= dalloc(s, &tgat, std::max<size_t>(P.tile_gat.size(), 1));
+  rc |= dalloc(s, &tdst, std::max<size_t>(P.tile_dst.size(), 1));
   rc |= dalloc(s, &bnd_off, N + 1);
   rc |= dalloc(s, &hslot, H);
   rc |= dalloc(s, &s->td.tpart, (size_t)P.ntiles);
@@ -218,7 +219,8 @@ template <typename R> static int alloc_impl(pbs_solver *s) {
   s->td.tiles = tiles;
   s->td.halo = halo;
   s->td.tet_lp = (const uint4 *)tlp;
-  s->td.tile_off = toff;
+  s->td.tile_gat = tgat;
+  s->td.tile_dst = tdst;
   s->td.bnd_off = bnd_off;
   s->td.halo_slot = hslot;
   rc |= dalloc(s, &m.ctrl, (size_t)s->ninst);
@@ -286,7 +288,10 @@ template <typename R> static int setup_impl(pbs_solver *s, const int *mask) {
     HIPCHK(hipMemcpyAsync((void *)s->td.halo, P.halo.data(), sizeof(int) * P.halo.size(), hipMemcpyHostToDevice, s->st));
   if (s->nt > 0)
     HIPCHK(hipMemcpyAsync((void *)s->td.tet_lp, P.tet_lp.data(), sizeof(uint32_t) * P.tet_lp.size(), hipMemcpyHostToDevice, s->st));
-  HIPCHK(hipMemcpyAsync((void *)s->td.tile_off, P.tile_off.data(), sizeof(uint16_t) * P.tile_off.size(), hipMemcpyHostToDevice, s->st));
+  if (!P.tile_gat.empty())
+    HIPCHK(hipMemcpyAsync((void *)s->td.tile_gat, P.tile_gat.data(), sizeof(uint32_t) * P.tile_gat.size(), hipMemcpyHostToDevice, s->st));
+  if (!P.tile_dst.empty())
+    HIPCHK(hipMemcpyAsync((void *)s->td.tile_dst, P.tile_dst.data(), sizeof(uint16_t) * P.tile_dst.size(), hipMemcpyHostToDevice, s->st));
   HIPCHK(hipMemcpyAsync((void *)s->td.bnd_off, P.bnd_off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, s->st));
   if (!P.halo_slot.empty())
     HIPCHK(hipMemcpyAsync((void *)s->td.halo_slot, P.halo_slot.data(), sizeof(int) * P.halo_slot.size(), hipMemcpyHostToDevice, s->st));
@@ -993,6 +998,43 @@ template <typename R> static int hd_impl(pbs_solver *s, const float *pos, float 
 extern "C" int pbs_compute_hessian_diag(pbs_solver *s, const float *pos, float *nh, float *th) {
   if (!s || !pos) return set_error(PBS_ERR_ARG, "null argument");
   return s->p.precision == PBS_FP64 ? hd_impl<double>(s, pos, nh, th) : hd_impl<float>(s, pos, nh, th);
+}
+
+template <typename R> static int tile_forces_impl(pbs_solver *s, const float *pos, double *nf, double *nh, double *en) {
+  DevMesh<R> &m = mesh<R>(s);
+  int rc = upload_planes<R>(s, pos, m.ring.b[0]);
+  if (rc) return rc;
+  DevParams dp = s->dp;
+  dp.f32math = 0;
+  Launch<R>::standalone_ctrl(m, 0, s->st, nh ? 1 : 0, 1);
+  Launch<R>::tile_pass(m, s->td, dp, s->st);
+  HIPCHK(hipGetLastError());
+  const size_t n3 = 3 * (size_t)s->nn;
+  for (int which = 0; which < 2; ++which) {
+    double *out = which == 0 ? nf : nh;
+    if (!out) continue;
+    Launch<R>::sum_partials(m, s->td, dp, which, s->stage64, s->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, s->stage64, sizeof(double) * n3, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+  }
+  if (en) {
+    std::vector<double> tp(std::max(s->td.ntiles, 1));
+    HIPCHK(hipMemcpyAsync(tp.data(), s->td.tpart, sizeof(double) * s->td.ntiles, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    double e = 0.0;
+    for (int b = 0; b < s->td.ntiles; ++b) e += tp[b];
+    *en = e;
+  }
+  HIPCHK(hipStreamSynchronize(s->st));
+  return PBS_OK;
+}
+
+extern "C" int pbs_tile_forces(pbs_solver *s, const float *pos, double *node_forces, double *node_hd, double *energy) {
+  if (!s || !pos) return set_error(PBS_ERR_ARG, "null argument");
+  if (s->ninst != 1) return set_error(PBS_ERR_STATE, "pbs_tile_forces: single-mesh handles only");
+  return s->p.precision == PBS_FP64 ? tile_forces_impl<double>(s, pos, node_forces, node_hd, energy)
+                                    : tile_forces_impl<float>(s, pos, node_forces, node_hd, energy);
 }
 
 template <typename R> static int rest_data_impl(pbs_solver *s, float *dminv, float *vol, float *mass, float *gforce) {

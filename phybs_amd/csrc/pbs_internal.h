@@ -55,8 +55,9 @@ struct TileDev {
   long halo_total;
   const TileInfo *tiles;
   const int *halo;
-  const uint4 *tet_lp;          // per tet: corner local ids, corner contribution positions (16 bits each)
-  const unsigned short *tile_off;
+  const uint4 *tet_lp;          // per tet: corner local ids, corner contribution slots (16 bits each)
+  const unsigned *tile_gat;     // per tile and gather rank: local id | run length << 16
+  const unsigned short *tile_dst;  // per tile: first slot of each jagged diagonal
   const int *bnd_off;           // per node: its halo partial slots [bnd_off[n], bnd_off[n+1]), tile order
   const int *halo_slot;         // per halo entry: its partial slot
   double *tpart;                // per tile: elastic energy of its tets
@@ -64,9 +65,11 @@ struct TileDev {
   double *fpart;                // per node block, mixed-mode FP32 passes only: |f|^2
 };
 
-// Values per corner contribution in the tile pass's LDS: 4 floats (one 16-byte access) or 3 doubles.
-template <typename R> struct ContribWidth { static constexpr int value = 3; };
-template <> struct ContribWidth<float> { static constexpr int value = 4; };
+// LDS of the tile pass, per local node (positions) and per contribution slot: FP32 one float4
+// (x, y, z, unused: one 16-byte access), FP64 a double2 plane (x, y) and a double plane (z), so
+// that every access is 16 or 8 bytes at a 16- or 8-byte stride.
+template <typename R> struct TileLdsBytes { static constexpr int value = 24; };
+template <> struct TileLdsBytes<float> { static constexpr int value = 16; };
 
 template <typename R> struct Ring {
   R *b[4];  // each: 3 planes of nn values (x | y | z)
@@ -115,7 +118,11 @@ template <typename R> struct Launch {
   static void control(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
   static void step_init(const DevMesh<R> &m, const TileDev &td, const DevParams &p, int bs, hipStream_t st);
   static void step_final(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
-  static void standalone_ctrl(const DevMesh<R> &m, int cur, hipStream_t st);
+  static void standalone_ctrl(const DevMesh<R> &m, int cur, hipStream_t st, int hd = 0, int chk = 0);
+  // per-node sums of the tile pass's partials (which: 0 forces, 1 Hessian diagonal), doubles in
+  // the caller's node order (3N AoS)
+  static void sum_partials(const DevMesh<R> &m, const TileDev &td, const DevParams &p, int which, double *out,
+                           hipStream_t st);
   static void gather(const DevMesh<R> &m, const DevParams &p, int bs, int which, void *out_planes, hipStream_t st);
   static void aos_to_planes(const float *aos, R *planes, const int *perm, int n, hipStream_t st);
   static void planes_to_aos(const R *planes, float *aos, const int *perm, int n, hipStream_t st);

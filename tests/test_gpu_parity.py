@@ -133,13 +133,14 @@ def run_pair(rest, tets, mask, x0, steps, gparams, oparams, precision):
 
 
 def test_positions_after_100_steps_c0():
-    """REF_GPU semantics, FP32 as the reference: 100 steps of 96 iterations from the twisted bar.
+    """REF_GPU semantics, FP32 as the reference: 100 steps of 96 iterations from the twisted bar,
+    positions within 1e-4 of the bar length of the FP32 oracle's.
 
     The reference sums per-node forces with float atomics (CUDAInterface.cu:526-541), so its own
-    FP32 trajectory depends on the summation order; the line search's accept/reject decisions
-    amplify that over 100 steps.  The bound is therefore the oracle's own order sensitivity
-    (ascending vs descending gather, both FP32), times 3, and never below 1e-4 L.  The
-    single-pass force and Hessian tests and the FP64 tracking test pin the arithmetic itself."""
+    FP32 trajectory depends on the summation order, and the line search's accept/reject decisions
+    amplify that over the steps; the oracle's own order sensitivity (ascending vs descending
+    gather, both FP32) is printed beside the error.  The single-pass force and Hessian tests and
+    the FP64 tracking test pin the arithmetic itself."""
     rest, tets, mask, x0 = bar_case()
     say("4x4x16 bar 100 steps REF_GPU fp32")
     xg, xo, gst, ost = run_pair(rest, tets, mask, x0, 100, P.default_params(), O.default_params(), 32)
@@ -152,8 +153,8 @@ def test_positions_after_100_steps_c0():
     err = np.abs(xg - xo).max()
     say("max |x_gpu - x_oracle| = %.2e L, oracle order noise %.2e L, rollbacks" % (err / L, noise / L),
         sum(g.rollbacks for g in gst), sum(q.rollbacks for q in ost))
-    assert err <= max(1e-4 * L, 3.0 * noise)
     assert np.isfinite(xg).all()
+    assert err <= 1e-4 * L
 
 
 def test_positions_ref_cpu_semantics():
