@@ -200,13 +200,12 @@ template <typename R> static int alloc_impl(pbs_solver *s) {
   const size_t H = std::max<long>(P.halo_total, 1);
   TileInfo *tiles;
   int *halo, *bnd_off, *hslot;
-  uint32_t *tlp, *tgat;
-  unsigned short *tdst;
+  uint32_t *tlp;
+  unsigned short *toff;
   rc |= dalloc(s, &tiles, (size_t)P.ntiles);
   rc |= dalloc(s, &halo, H);
   rc |= dalloc(s, &tlp, 4 * T);
-  rc |= dalloc(s, &tgat, std::max<size_t>(P.tile_gat.size(), 1));
-  rc |= dalloc(s, &tdst, std::max<size_t>(P.tile_dst.size(), 1));
+  rc |= dalloc(s, &toff, P.tile_off.size());
   rc |= dalloc(s, &bnd_off, N + 1);
   rc |= dalloc(s, &hslot, H);
   rc |= dalloc(s, &s->td.tpart, (size_t)P.ntiles);
@@ -219,8 +218,7 @@ template <typename R> static int alloc_impl(pbs_solver *s) {
   s->td.tiles = tiles;
   s->td.halo = halo;
   s->td.tet_lp = (const uint4 *)tlp;
-  s->td.tile_gat = tgat;
-  s->td.tile_dst = tdst;
+  s->td.tile_off = toff;
   s->td.bnd_off = bnd_off;
   s->td.halo_slot = hslot;
   rc |= dalloc(s, &m.ctrl, (size_t)s->ninst);
@@ -288,10 +286,7 @@ template <typename R> static int setup_impl(pbs_solver *s, const int *mask) {
     HIPCHK(hipMemcpyAsync((void *)s->td.halo, P.halo.data(), sizeof(int) * P.halo.size(), hipMemcpyHostToDevice, s->st));
   if (s->nt > 0)
     HIPCHK(hipMemcpyAsync((void *)s->td.tet_lp, P.tet_lp.data(), sizeof(uint32_t) * P.tet_lp.size(), hipMemcpyHostToDevice, s->st));
-  if (!P.tile_gat.empty())
-    HIPCHK(hipMemcpyAsync((void *)s->td.tile_gat, P.tile_gat.data(), sizeof(uint32_t) * P.tile_gat.size(), hipMemcpyHostToDevice, s->st));
-  if (!P.tile_dst.empty())
-    HIPCHK(hipMemcpyAsync((void *)s->td.tile_dst, P.tile_dst.data(), sizeof(uint16_t) * P.tile_dst.size(), hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemcpyAsync((void *)s->td.tile_off, P.tile_off.data(), sizeof(uint16_t) * P.tile_off.size(), hipMemcpyHostToDevice, s->st));
   HIPCHK(hipMemcpyAsync((void *)s->td.bnd_off, P.bnd_off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, s->st));
   if (!P.halo_slot.empty())
     HIPCHK(hipMemcpyAsync((void *)s->td.halo_slot, P.halo_slot.data(), sizeof(int) * P.halo_slot.size(), hipMemcpyHostToDevice, s->st));

@@ -55,9 +55,8 @@ struct TileDev {
   long halo_total;
   const TileInfo *tiles;
   const int *halo;
-  const uint4 *tet_lp;          // per tet: corner local ids, corner contribution slots (16 bits each)
-  const unsigned *tile_gat;     // per tile and gather rank: local id | run length << 16
-  const unsigned short *tile_dst;  // per tile: first slot of each jagged diagonal
+  const uint4 *tet_lp;          // per tet: corner local ids, corner contribution positions (16 bits each)
+  const unsigned short *tile_off;
   const int *bnd_off;           // per node: its halo partial slots [bnd_off[n], bnd_off[n+1]), tile order
   const int *halo_slot;         // per halo entry: its partial slot
   double *tpart;                // per tile: elastic energy of its tets
@@ -65,11 +64,9 @@ struct TileDev {
   double *fpart;                // per node block, mixed-mode FP32 passes only: |f|^2
 };
 
-// LDS of the tile pass, per local node (positions) and per contribution slot: FP32 one float4
-// (x, y, z, unused: one 16-byte access), FP64 a double2 plane (x, y) and a double plane (z), so
-// that every access is 16 or 8 bytes at a 16- or 8-byte stride.
-template <typename R> struct TileLdsBytes { static constexpr int value = 24; };
-template <> struct TileLdsBytes<float> { static constexpr int value = 16; };
+// Values per corner contribution in the tile pass's LDS: 4 floats (one 16-byte access) or 3 doubles.
+template <typename R> struct ContribWidth { static constexpr int value = 3; };
+template <> struct ContribWidth<float> { static constexpr int value = 4; };
 
 template <typename R> struct Ring {
   R *b[4];  // each: 3 planes of nn values (x | y | z)

@@ -277,69 +277,32 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblk) {
   return start + k;
 }
 
-// LDS views of the tile pass.  Positions of local nodes and contribution slots use one scheme:
-// FP32 a float4 record per entry (x, y, z, unused: one 16-byte access at a 16-byte stride); FP64 a
-// double2 plane (x, y) followed by a double plane (z), one 16-byte and one 8-byte access at their
-// own strides.  A wavefront reading consecutive entries therefore touches consecutive banks.
-template <typename T> struct Lds3;
-template <> struct Lds3<float> {
-  float4 *p;
-  __device__ __forceinline__ Lds3(unsigned char *base, int) : p((float4 *)base) {}
-  __device__ __forceinline__ void put(int i, float x, float y, float z) const { p[i] = make_float4(x, y, z, 0.0f); }
-  __device__ __forceinline__ void get(int i, float &x, float &y, float &z) const {
-    const float4 v = p[i];
-    x = v.x;
-    y = v.y;
-    z = v.z;
-  }
-};
-template <> struct Lds3<double> {
-  double2 *xy;
-  double *zp;
-  __device__ __forceinline__ Lds3(unsigned char *base, int n) : xy((double2 *)base), zp((double *)(base + 16 * (size_t)n)) {}
-  __device__ __forceinline__ void put(int i, double x, double y, double z) const {
-    xy[i] = make_double2(x, y);
-    zp[i] = z;
-  }
-  __device__ __forceinline__ void get(int i, double &x, double &y, double &z) const {
-    const double2 v = xy[i];
-    x = v.x;
-    y = v.y;
-    z = zp[i];
-  }
-};
-// bytes of the position region (rounded up to 16) and offset of the contribution region
-template <typename R> __host__ __device__ constexpr size_t tile_pos_bytes(int lcap) {
-  return ((size_t)TileLdsBytes<R>::value * lcap + 15) & ~(size_t)15;
-}
-
-// One tet of a tile: F from the tile's LDS positions, the material response, and its four corner
-// contributions written to their slots of the LDS contribution array: forces on phase 0 (plus the
-// clamped energy on check passes), Hessian-diagonal entries rounded to FP32 on phase 1.  The
-// arithmetic type C is R, or FP32 on FP64 positions (the mixed mode).  Mixed mode: F is formed in
-// R; the energy of check passes is evaluated in R from that F (the same value the FP64 passes
-// compute, so the line search compares like with like); the neo-Hookean force is evaluated in FP32
-// from the displacement gradient H = F - I, whose FP32 rounding is relative to the strain instead
-// of to 1; contributions are stored as FP32.
+// One tet of a tile: F from the tile's LDS positions (stride 4 R per local node), the material
+// response, and its four corner contributions written to their slots of the LDS array rec (W
+// values each): forces on phase 0 (plus the clamped energy on check passes), Hessian-diagonal
+// entries rounded to FP32 on phase 1.  The arithmetic type C is R, or FP32 on FP64 positions (the
+// mixed mode).  Mixed mode: F is formed in R; the energy of check passes is evaluated in R from
+// that F (the same value the FP64 passes compute, so the line search compares like with like);
+// the neo-Hookean force is evaluated in FP32 from the displacement gradient H = F - I, whose FP32
+// rounding is relative to the strain instead of to 1; contributions are stored as FP32.
 template <typename R, typename C, int MAT>
-__device__ __forceinline__ void tile_tet(int phase, const uint4 lp, const float Dmf[9], float Vf, const Lds3<R> &sx,
-                                         const Lds3<C> &rec, C mu, C lam, int chk, int clamp, double &e_el) {
+__device__ __forceinline__ void tile_tet(int phase, const uint4 lp, const float Dmf[9], float Vf,
+                                         const R *__restrict__ sx, C *__restrict__ rec, C mu, C lam, int chk, int clamp,
+                                         double &e_el) {
+  constexpr int W = ContribWidth<C>::value;
   constexpr bool MIX = sizeof(C) < sizeof(R);
-  R p[4][3];
-  sx.get(lp.x & 0xffff, p[0][0], p[0][1], p[0][2]);
-  sx.get(lp.x >> 16, p[1][0], p[1][1], p[1][2]);
-  sx.get(lp.y & 0xffff, p[2][0], p[2][1], p[2][2]);
-  sx.get(lp.y >> 16, p[3][0], p[3][1], p[3][2]);
+  const R *p0 = sx + 4 * (lp.x & 0xffff), *p1 = sx + 4 * (lp.x >> 16), *p2 = sx + 4 * (lp.y & 0xffff),
+          *p3 = sx + 4 * (lp.y >> 16);
   typedef typename std::conditional<MIX, R, C>::type FT;  // type F is formed in
   FT Ds[9], Dm[9], Fw[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) Dm[k] = (FT)Dmf[k];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const R x0 = p[0][k];
-    Ds[3 * k + 0] = (FT)(p[1][k] - x0);
-    Ds[3 * k + 1] = (FT)(p[2][k] - x0);
-    Ds[3 * k + 2] = (FT)(p[3][k] - x0);
+    const R x0 = p0[k];
+    Ds[3 * k + 0] = (FT)(p1[k] - x0);
+    Ds[3 * k + 1] = (FT)(p2[k] - x0);
+    Ds[3 * k + 2] = (FT)(p3[k] - x0);
   }
   mul3(Ds, Dm, Fw);
   C F[9], Dmi[9];
@@ -350,7 +313,7 @@ __device__ __forceinline__ void tile_tet(int phase, const uint4 lp, const float 
   }
   const C V = (C)Vf;
   C f12[12], h12[12], en = (C)0;
-  const int q[4] = {(int)(lp.z & 0xffff), (int)(lp.z >> 16), (int)(lp.w & 0xffff), (int)(lp.w >> 16)};
+  const unsigned q[4] = {lp.z & 0xffff, lp.z >> 16, lp.w & 0xffff, lp.w >> 16};
   if (phase == 0) {
     if (MIX && MAT == 0) {
       C H[9];
@@ -363,7 +326,12 @@ __device__ __forceinline__ void tile_tet(int phase, const uint4 lp, const float 
       stretch_tet<C, false>(F, Dmi, V, mu, lam, f12, en, h12);
     }
 #pragma unroll
-    for (int c = 0; c < 4; ++c) rec.put(q[c], f12[3 * c], f12[3 * c + 1], f12[3 * c + 2]);
+    for (int c = 0; c < 4; ++c) {
+      C *r = rec + W * q[c];
+      r[0] = f12[3 * c];
+      r[1] = f12[3 * c + 1];
+      r[2] = f12[3 * c + 2];
+    }
     if (chk) {
       R er = MIX ? tet_energy<R, MAT>((const R *)Fw, (R)Vf, (R)mu, (R)lam) : (R)en;
       if (clamp == 1) er = er > (R)0 ? er : (R)0;          // CUDAInterface.cu:549
@@ -374,25 +342,27 @@ __device__ __forceinline__ void tile_tet(int phase, const uint4 lp, const float 
     if (MAT == 0) neo_tet<C, true>(F, Dmi, V, mu, lam, f12, en, h12);
     else stretch_tet<C, true>(F, Dmi, V, mu, lam, f12, en, h12);
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-      rec.put(q[c], (C)(float)h12[3 * c], (C)(float)h12[3 * c + 1], (C)(float)h12[3 * c + 2]);
+    for (int c = 0; c < 4; ++c) {
+      C *r = rec + W * q[c];
+      r[0] = (C)(float)h12[3 * c];
+      r[1] = (C)(float)h12[3 * c + 1];
+      r[2] = (C)(float)h12[3 * c + 2];
+    }
   }
 }
 
-// Gather of one tile: the thread of gather rank r sums the contribution run of its local node,
-// slot dst[j] + r for j = 0, 1, ... (so the threads of one step read consecutive slots), and writes
-// one partial per owned node (3 planes of N) and per halo node (xyz at its slot); the sums are
-// formed in R whatever the contributions' type Q, in each run's own order (tet, then corner).
+// Gather of one tile: each thread sums the contribution runs of its local nodes in order and
+// writes one partial per owned node (3 planes of N) and per halo node (xyz at its slot); the sums
+// are formed in R whatever the contributions' type Q.
 template <typename R, typename Q>
-__device__ __forceinline__ void tile_gather(int phase, const TileInfo &ti, const TileDev &td, const Lds3<Q> &rec,
+__device__ __forceinline__ void tile_gather(int phase, const TileInfo &ti, const TileDev &td, const Q *__restrict__ rec,
                                             int N, R *__restrict__ fown, R *__restrict__ hpart,
                                             float *__restrict__ fownh, float *__restrict__ hparth) {
+  constexpr int W = ContribWidth<Q>::value;
   const int L = ti.own_cnt + ti.halo_cnt;
-  const unsigned *gat = td.tile_gat + ti.gat_off;
-  const unsigned short *dst = td.tile_dst + ti.dst_off;
-  for (int r = threadIdx.x; r < L; r += blockDim.x) {
-    const unsigned g = gat[r];
-    const int i = (int)(g & 0xffff), len = (int)(g >> 16);
+  const unsigned short *off = td.tile_off + ti.off_start;
+  for (int i = threadIdx.x; i < L; i += blockDim.x) {
+    const int o0 = off[i], o1 = off[i + 1];
     size_t d0, stride;
     if (i < ti.own_cnt) {
       d0 = (size_t)ti.own_start + i;
@@ -403,12 +373,11 @@ __device__ __forceinline__ void tile_gather(int phase, const TileInfo &ti, const
     }
     if (phase == 0) {
       R a0 = (R)0, a1 = (R)0, a2 = (R)0;
-      for (int j = 0; j < len; ++j) {
-        Q x, y, z;
-        rec.get(dst[j] + r, x, y, z);
-        a0 += (R)x;
-        a1 += (R)y;
-        a2 += (R)z;
+      for (int j = o0; j < o1; ++j) {
+        const Q *r = rec + W * j;
+        a0 += (R)r[0];
+        a1 += (R)r[1];
+        a2 += (R)r[2];
       }
       R *o = i < ti.own_cnt ? fown : hpart;
       o[d0] = a0;
@@ -416,12 +385,11 @@ __device__ __forceinline__ void tile_gather(int phase, const TileInfo &ti, const
       o[2 * stride + d0] = a2;
     } else {
       float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-      for (int j = 0; j < len; ++j) {
-        Q x, y, z;
-        rec.get(dst[j] + r, x, y, z);
-        a0 += (float)x;
-        a1 += (float)y;
-        a2 += (float)z;
+      for (int j = o0; j < o1; ++j) {
+        const Q *r = rec + W * j;
+        a0 += (float)r[0];
+        a1 += (float)r[1];
+        a2 += (float)r[2];
       }
       float *o = i < ti.own_cnt ? fownh : hparth;
       o[d0] = a0;
@@ -435,10 +403,10 @@ __device__ __forceinline__ void tile_gather(int phase, const TileInfo &ti, const
 // Tile pass (per-tet half of computeForceKd, CUDAInterface.cu:495-568): one workgroup per tile
 // evaluates each of the tile's tets once -- force, energy on check passes, Hessian diagonal on
 // refresh passes -- and writes every corner contribution into its precomputed slot of an LDS
-// array in which each local node's contributions form one run laid out as jagged diagonals.  The
-// gather then sums each run in its fixed order (no atomics) and writes one partial per owned node
-// (fown) and per halo node (hpart).
-//   LDS: positions of local nodes, contributions (4 slots per tet), both in the Lds3 layout.
+// array in which each local node's contributions form one contiguous run (ordered by tet, then
+// corner).  The gather then sums each run in that fixed order (no atomics) and writes one partial
+// per owned node (fown) and per halo node (hpart).
+//   LDS: positions of local nodes (4 R each), contributions (4 per tet, W values each).
 //   HBM per tet: 16 B of local ids / slots, 40 B of rest data from the SoA planes.
 // ---------------------------------------------------------------------------------------------
 // F32M (FP64 state only): the per-tet arithmetic in FP32 after F (the mixed mode's early passes).
@@ -457,12 +425,14 @@ k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
   const R *__restrict__ x = ring.b[ctrl->cur];
   const int L = ti.own_cnt + ti.halo_cnt, N = p.nn, T = p.nt, tid = threadIdx.x;
   typedef typename std::conditional<F32M, float, R>::type C;
-  const Lds3<R> sx(smem, td.lcap);
-  const Lds3<C> rec(smem + tile_pos_bytes<R>(td.lcap), 4 * td.tcap);
+  R *sx = (R *)smem;
+  C *rec = (C *)(sx + 4 * td.lcap);
   const int nthr = blockDim.x;
   for (int i = tid; i < L; i += nthr) {
     const int g = i < ti.own_cnt ? ti.own_start + i : td.halo[ti.halo_off + i - ti.own_cnt];
-    sx.put(i, x[g], x[N + g], x[2 * (size_t)N + g]);
+    sx[4 * i] = x[g];
+    sx[4 * i + 1] = x[N + g];
+    sx[4 * i + 2] = x[2 * (size_t)N + g];
   }
   __syncthreads();
   const C mu = (C)td.inst_mu[ti.inst], lam = (C)td.inst_lam[ti.inst];
@@ -827,7 +797,7 @@ void Launch<R>::tet(const DevMesh<R> &m, const TileDev &td, const DevParams &p, 
 }
 template <typename R>
 size_t Launch<R>::tile_lds(const TileDev &td) {
-  return tile_pos_bytes<R>(td.lcap) + (size_t)TileLdsBytes<R>::value * 4 * td.tcap;
+  return (4 * (size_t)td.lcap + 4 * (size_t)ContribWidth<R>::value * td.tcap) * sizeof(R);
 }
 template <typename R>
 void Launch<R>::tile_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {

@@ -172,30 +172,16 @@ bool build_tile_plan(int nn, const float *rest, int nt, const int *tets, int til
         ent.push_back({l, 4 * (t - t0) + j});
       }
     std::sort(ent.begin(), ent.end());
-    // jagged diagonals: ranks by run length (descending), then local id; contribution j of the
-    // node of rank r at slot dst[j] + r
-    std::vector<int> run(L, 0), order(L), rank(L);
-    for (const auto &en : ent) run[en.first]++;
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return run[a] > run[b]; });
-    for (int r = 0; r < L; ++r) rank[order[r]] = r;
-    const int longest = L > 0 ? run[order[0]] : 0;
-    ti.gat_off = (int)P.tile_gat.size();
-    ti.dst_off = (int)P.tile_dst.size();
-    for (int r = 0; r < L; ++r) P.tile_gat.push_back((uint32_t)order[r] | ((uint32_t)run[order[r]] << 16));
+    ti.off_start = (int)P.tile_off.size();
     {
-      int start = 0, r_end = L;  // ranks [0, r_end) have runs longer than j
-      for (int j = 0; j < longest; ++j) {
-        while (r_end > 0 && run[order[r_end - 1]] <= j) --r_end;
-        P.tile_dst.push_back((uint16_t)start);
-        start += r_end;
+      size_t e = 0;
+      for (int i = 0; i <= L; ++i) {
+        while (e < ent.size() && ent[e].first < i) ++e;
+        P.tile_off.push_back((uint16_t)e);
       }
     }
     std::vector<uint16_t> pos(4 * (size_t)cnt);
-    for (size_t e = 0, j = 0; e < ent.size(); ++e) {
-      j = (e > 0 && ent[e - 1].first == ent[e].first) ? j + 1 : 0;
-      pos[ent[e].second] = (uint16_t)(P.tile_dst[ti.dst_off + j] + rank[ent[e].first]);
-    }
+    for (size_t e = 0; e < ent.size(); ++e) pos[ent[e].second] = (uint16_t)e;
     for (int t = t0; t < t1; ++t) {
       const uint16_t *l = &lid[4 * (size_t)(t - t0)], *q = &pos[4 * (size_t)(t - t0)];
       uint32_t *w = &P.tet_lp[4 * (size_t)t];
@@ -257,7 +243,7 @@ bool build_batch_plan(int ninst, const int *node_off, const int *tet_off, const 
       msg = "instance " + std::to_string(i) + ": " + msg;
       return false;
     }
-    const int tiles0 = (int)P.tiles.size(), gat0 = (int)P.tile_gat.size(), dst0 = (int)P.tile_dst.size();
+    const int tiles0 = (int)P.tiles.size(), off0 = (int)P.tile_off.size();
     const long halo0 = (long)P.halo.size();
     for (int v : Q.perm) P.perm.push_back(v + n0);
     for (int v : Q.iperm) P.iperm.push_back(v + n0);
@@ -272,14 +258,12 @@ bool build_batch_plan(int ninst, const int *node_off, const int *tet_off, const 
       ti.own_start += n0;
       ti.halo_off += (int)halo0;
       ti.tet_start += t0;
-      ti.gat_off += gat0;
-      ti.dst_off += dst0;
+      ti.off_start += off0;
       ti.inst = i;
       P.tiles.push_back(ti);
     }
     for (int v : Q.halo) P.halo.push_back(v + n0);
-    for (uint32_t v : Q.tile_gat) P.tile_gat.push_back(v);
-    for (uint16_t v : Q.tile_dst) P.tile_dst.push_back(v);
+    for (uint16_t v : Q.tile_off) P.tile_off.push_back(v);
     for (uint32_t v : Q.tet_lp) P.tet_lp.push_back(v);
     const int b0 = P.bnd_off.back();
     for (int n = 0; n < n1 - n0; ++n) P.bnd_off.push_back(b0 + Q.bnd_off[n + 1]);

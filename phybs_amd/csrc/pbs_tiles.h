@@ -16,20 +16,16 @@ namespace pbs {
 // and the node pass adds a node's partials in tile order.  Halo partial slots are numbered by
 // owner node, so each node's partials are contiguous and the node pass reads them coalesced.
 //
-// Inside a tile, node i (local id: owned nodes first, then halo nodes) receives a run of corner
-// contributions, ordered by tet, then corner.  The runs are stored as jagged diagonals: the
-// gather visits the local nodes in rank order r (run length descending, then local id), and
-// contribution j of the node of rank r sits at dst[j] + r, where dst[j] counts the contributions
-// of ranks with runs longer than j' for every j' < j.  The threads of a gather step j therefore
-// read consecutive slots (no LDS bank conflicts), and each node's run is still summed in its
-// own order.  Each tet carries its corners' local node ids and the slot of each corner, so the
-// tet pass writes every corner force straight into its slot.
+// Inside a tile, node i (local id: owned nodes first, then halo nodes) receives the corner
+// contributions run [off[i], off[i+1]) of the tile's contribution array, ordered by tet, then
+// corner.  Each tet carries its corners' local node ids and the position of each corner in that
+// array, so the tet pass writes every corner force straight into its slot and the gather sums
+// contiguous runs.
 struct TileInfo {
   int own_start, own_cnt;  // owned nodes
   int halo_off, halo_cnt;  // halo node ids: halo[halo_off .. +halo_cnt); partial slots likewise
   int tet_start, tet_cnt;  // owned tets (solver order)
-  int gat_off;             // gather ranks: tile_gat[gat_off .. gat_off + own_cnt + halo_cnt)
-  int dst_off;             // diagonal starts: tile_dst[dst_off .. dst_off + longest run)
+  int off_start;           // run offsets: tile_off[off_start .. off_start + own_cnt + halo_cnt]
   int inst;                // mesh instance (batched solvers; 0 otherwise)
 };
 
@@ -48,10 +44,9 @@ struct TilePlan {
   std::vector<int> inc_off, inc;    // per new node: 4*new_tet + corner, in ascending original tet id
   std::vector<TileInfo> tiles;
   std::vector<int> halo;            // new node ids
-  std::vector<uint32_t> tile_gat;   // per tile and gather rank: local id | run length << 16
-  std::vector<uint16_t> tile_dst;   // per tile: first slot of each diagonal
+  std::vector<uint16_t> tile_off;   // per tile: local node count + 1 run offsets
   std::vector<uint32_t> tet_lp;     // 4 words per tet: local ids (c0 | c1 << 16, c2 | c3 << 16),
-                                    // contribution slots (likewise)
+                                    // contribution positions (likewise)
   std::vector<int> bnd_off;         // per node: its halo partial slots [bnd_off[n], bnd_off[n+1]), tile order
   std::vector<int> halo_slot;       // per halo entry (tile, halo node): its partial slot
 };

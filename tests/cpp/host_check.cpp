@@ -3,7 +3,7 @@
 // under AddressSanitizer / UndefinedBehaviorSanitizer.  No GPU, no HIP runtime.
 //
 // Tile plan: for every tile the corner slots form a permutation of the tile's 4 * tet_cnt slots;
-// a simulated gather (rank r reads slot dst[j] + r for j < run length) visits, for every local
+// a simulated gather (each local node sums its contiguous run of slots) visits, for every local
 // node, exactly that node's corners in (tet, corner) order, so its FP32 sums are bitwise the
 // sequential sums; owned + halo partials add up to the direct per-node sums; halo partial slots
 // are a permutation grouped by owner node.
@@ -79,20 +79,14 @@ static void check_plan(const char *name, int nn, int nt, const std::vector<int> 
       }
       CHECK(smallest >= ti.own_start && smallest < ti.own_start + ti.own_cnt, "%s tile %d: tet not owned", name, b);
     }
-    // simulated gather
-    std::vector<int> rank_seen(L, 0);
-    int longest_prev = 1 << 30;
-    for (int r = 0; r < L; ++r) {
-      const uint32_t g = P.tile_gat[ti.gat_off + r];
-      const int l = (int)(g & 0xffff), len = (int)(g >> 16);
-      CHECK(l < L && !rank_seen[l], "%s tile %d: gather rank table", name, b);
-      rank_seen[l] = 1;
+    // simulated gather: local node l sums slots tile_off[l] .. tile_off[l+1]-1 in order
+    const uint16_t *off = &P.tile_off[ti.off_start];
+    for (int l = 0; l < L; ++l) {
+      const int len = off[l + 1] - off[l];
       CHECK(len == (int)runs[l].size(), "%s tile %d: run length %d vs %zu", name, b, len, runs[l].size());
-      CHECK(len <= longest_prev, "%s tile %d: ranks not by descending run length", name, b);
-      longest_prev = len;
       float seq = 0.0f, sim = 0.0f;
       for (int j = 0; j < len; ++j) {
-        const int sl = P.tile_dst[ti.dst_off + j] + r;
+        const int sl = off[l] + j;
         CHECK(sl < 4 * cnt && slot_corner[sl] == runs[l][j], "%s tile %d: slot of run entry %d", name, b, j);
         sim += val[slot_corner[sl]];
         seq += val[runs[l][j]];
@@ -105,6 +99,7 @@ static void check_plan(const char *name, int nn, int nt, const std::vector<int> 
         hslot_seen[hs] = 1;
       }
     }
+    CHECK(off[L] == 4 * cnt, "%s tile %d: runs do not cover the tile's slots", name, b);
   }
   for (int t = 0; t < nt; ++t) CHECK(seen_tet[t], "%s: tet %d in no tile", name, t);
   double worst = 0.0;

@@ -107,15 +107,23 @@ def test_bar64_tile_pass_node_forces_fp32(bar64):
     assert abs(e - eo) <= 1e-6 * abs(eo)
 
 
+def c1_case():
+    rest, tets = M.bar(12, 12, 50, 0.05)
+    return rest, tets, M.clamp_x0(rest), M.twisted_state(rest, 12, 12, 50, 0.05, 90.0, 0.01, 1)
+
+
 def test_bar12x12x50_500_fp32_steps():
     """The 12x12x50 bar released from a 90 degree twist with 1 % jitter: 500 FP32 steps of 96
-    iterations (dt = 1/30, gravity -z), REF_GPU semantics, against the FP32 oracle.  The final
-    positions within 1e-4 of the bar length (2.5 m).  Printed beside it at checkpoints: the FP32
-    oracle against itself with the opposite per-node summation order, i.e. how far the
-    reference's own FP32 trajectory moves with the order its float atomics happen to take."""
-    rest, tets = M.bar(12, 12, 50, 0.05)
-    mask = M.clamp_x0(rest)
-    x0 = M.twisted_state(rest, 12, 12, 50, 0.05, 90.0, 0.01, 1)
+    iterations (dt = 1/30, gravity -z), REF_GPU semantics, against the FP32 oracle.
+
+    After 100 steps the positions agree within 1e-4 of the bar length (2.5 m).  Beyond that the
+    FP32 trajectory of this configuration depends on the summation order itself: the reference
+    sums node forces with float atomics (CUDAInterface.cu:526-541), and the FP32 oracle against
+    itself with the opposite per-node order drifts to ~5e-4 L by step 200 as the bar settles and
+    the line search rolls back hundreds of times.  Over steps 100-500 the GPU's distance from the
+    oracle therefore stays within 3 times the largest such order sensitivity seen at the same
+    checkpoints; the FP64 test below holds the arithmetic over the same 500 steps."""
+    rest, tets, mask, x0 = c1_case()
     L = 2.5
     say("12x12x50 bar, 500 FP32 steps")
     s = P.Solver(rest, tets, mask, P.default_params(dt=1.0 / 30.0, grav=GRAV_Z, profile=BENCH_PROFILE))
@@ -127,18 +135,51 @@ def test_bar12x12x50_500_fp32_steps():
         o.set_state(x=x0.astype(np.float64))
         orcs.append(o)
     rb = [0, 0]
+    errs, noises = {}, {}
     for k in range(1, 501):
         rb[0] += s.step().rollbacks
         rb[1] += orcs[0].step().rollbacks
         orcs[1].step()
-        if k in (1, 10, 100, 200, 300, 400, 500):
+        if k in (1, 10, 50, 100, 150, 200, 250, 300, 400, 500):
             xg, xo, xr = s.positions(f64=True), orcs[0].state()[0], orcs[1].state()[0]
+            assert np.isfinite(xg).all()
+            errs[k], noises[k] = np.abs(xg - xo).max() / L, np.abs(xr - xo).max() / L
             say("step %3d: |x_gpu - x_oracle| %.2e L, oracle order noise %.2e L, rollbacks gpu %d oracle %d" %
-                (k, np.abs(xg - xo).max() / L, np.abs(xr - xo).max() / L, rb[0], rb[1]))
-    err = np.abs(xg - xo).max()
+                (k, errs[k], noises[k], rb[0], rb[1]))
     s.close()
-    assert np.isfinite(xg).all()
-    assert err <= 1e-4 * L
+    assert errs[100] <= 1e-4
+    late = [k for k in errs if k >= 100]
+    assert max(errs[k] for k in late) <= 3.0 * max(noises[k] for k in late)
+
+
+def test_bar12x12x50_500_fp64_steps():
+    """The same 500 steps in FP64 against the FP64 oracle: positions within 1e-6 of the bar length
+    at every checkpoint, and the same rollbacks while the bar still moves (the first 250 steps).
+    Once it has come to rest, successive energies differ only by rounding, so the line search's
+    accept/rollback decisions depend on the order of the energy sums (the GPU's fixed-order FP64
+    reduction against the oracle's sequential sum) and the rollback counts part."""
+    rest, tets, mask, x0 = c1_case()
+    L = 2.5
+    say("12x12x50 bar, 500 FP64 steps")
+    s = P.Solver(rest, tets, mask, P.default_params(dt=1.0 / 30.0, grav=GRAV_Z, profile=BENCH_PROFILE,
+                                                   precision=P.FP64))
+    s.set_positions(x0)
+    o = O.Solver(rest, tets, mask, params=O.default_params(dt=1.0 / 30.0, grav=GRAV_Z, profile=BENCH_PROFILE),
+                 precision=64, threads=threads())
+    o.set_state(x=x0.astype(np.float64))
+    rb = [0, 0]
+    worst = 0.0
+    for k in range(1, 501):
+        rb[0] += s.step().rollbacks
+        rb[1] += o.step().rollbacks
+        if k % 50 == 0:
+            err = np.abs(s.positions(f64=True) - o.state()[0]).max() / L
+            worst = max(worst, err)
+            say("step %3d: |x_gpu - x_oracle| %.2e L, rollbacks gpu %d oracle %d" % (k, err, rb[0], rb[1]))
+        if k == 250:
+            assert rb[0] == rb[1]
+    s.close()
+    assert worst <= 1e-6
 
 
 def batch_instances():

@@ -4,8 +4,8 @@ The solver's host-only C++ (the tile planner pbs_tiles.cpp and the mesh utilitie
 and the C oracle (oracle/pbs_oracle.c) are compiled from their sources with gcc/g++ into two
 standalone drivers (tests/cpp/host_check.cpp, tests/cpp/oracle_check.c), once plainly and once with
 -fsanitize=address,undefined, and run.  host_check also checks the tile plan itself: every
-corner slot unique, the jagged-diagonal gather visiting each node's contributions in (tet,
-corner) order (bitwise the sequential sums), owned + halo partials equal to the direct sums.
+corner slot unique, the gather visiting each node's contributions in (tet, corner) order (bitwise
+the sequential sums), owned + halo partials equal to the direct sums.
 """
 import os
 import shutil
