@@ -26,8 +26,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (each scaled by the
 # same counter's reading for a plain elementwise kernel of known size) on the 64x64x256 bar:
-# (kernel, bytes per position component) -> bytes.  Commands and raw figures:
-# profiles/k_tile_k_node_bar64x64x256_relgrad_f64.md.
+# (kernel, bytes per position component) -> bytes.  These are not measured by this script (a
+# counter run serialises every dispatch); the bench line names the profile they come from.
+TRAFFIC_PROFILE = "profiles/k_tile_k_node_bar64x64x256_relgrad_f64.md"
 MEASURED_TRAFFIC = {("k_tile", 8): 476.6e6, ("k_tile", 4): 427.3e6, ("k_node", 8): 202.3e6, ("k_node", 4): 118.5e6}
 
 WORKLOADS = {
@@ -69,40 +70,46 @@ def algorithmic_bytes(nt, nn, rsz):
     return {"k_tile": t, "k_tile_mixed": t, "k_node": (15.0 * rsz + 20.0) * nn}
 
 
-def cpu_baseline(rest, tets, mask, material, iters_per_step, grav, seconds=15.0):
-    """The oracle (FP64, OpenMP) timed on the host on a bounded sample: a fixed number of descent
-    iterations of the same mesh, scaled to steps/s with the GPU's iterations per step.  The
-    sample size comes from a 16-iteration probe (which, like the sample, includes one Hessian
-    refresh per 32 iterations at most)."""
+def host_threads():
+    """Threads for the CPU baseline: the cores this process may run on, at most 16 (one GPU's
+    share of the host)."""
+    return min(16, len(os.sched_getaffinity(0)))
+
+
+def cpu_baseline(rest, tets, mask, material, grav, gpu_fixed_sps=None, relgrad=False):
+    """The CPU restatement of the reference's descent path (the oracle; the reference's own CPU
+    code cannot be built here) timed on the host on one full step of the 64x64x256 bar's own
+    configuration: 64 FP32 descent iterations from rest (Hessian refreshes at iterations 0 and 32
+    included), on all host threads; nothing extrapolated.  With relgrad, also one full FP64
+    rel-grad 1e-4 step (about 2 minutes on 16 threads, so off by default)."""
     from oracle import pyoracle as O
 
-    cores = min(16, len(os.sched_getaffinity(0)))
-    p = O.default_params(iterations=16, grav=grav, dt=1.0 / 30.0,
-                         profile=[(1, 0.96), (7, 0.991), (10, 0.9995)])
-    o = O.Solver(rest, tets, mask, params=p, precision=64, material=material, threads=cores)
-    t0 = time.perf_counter()
-    o.step()
-    t1 = time.perf_counter()
-    per_iter = (t1 - t0) / 16.0
-    n = max(16, min(400, int(seconds / max(per_iter, 1e-9))))
-    p.iterations = n
-    o.set_params(p)
+    cores = host_threads()
+    prof = [(1, 0.96), (7, 0.991), (10, 0.9995)]
+    p = O.default_params(iterations=64, grav=grav, dt=1.0 / 30.0, profile=prof)
+    o = O.Solver(rest, tets, mask, params=p, precision=32, material=material, threads=cores)
     o.set_state(x=rest.astype(np.float64))
     t0 = time.perf_counter()
-    o.step()
-    dt = time.perf_counter() - t0
-    its = n / dt
-    # one thread (the reference's CPU descent is serial): 8 iterations, reported beside the value
-    p.iterations = 8
-    o1 = O.Solver(rest, tets, mask, params=p, precision=64, material=material, threads=1)
-    t0 = time.perf_counter()
-    o1.step()
-    its1 = 8 / (time.perf_counter() - t0)
-    return {"value": its / max(iters_per_step, 1.0), "unit": "steps/s", "cores": cores, "kind": "port",
-            "sample": f"{n} FP64 descent iterations of the same mesh in {dt:.1f} s "
-                      f"({its:.2f} it/s), divided by the GPU's {iters_per_step:.0f} iterations/step",
-            "iterations_per_s": its, "single_thread_iterations_per_s": its1,
-            "single_thread_steps_per_s": its1 / max(iters_per_step, 1.0)}
+    st = o.step()
+    el = time.perf_counter() - t0
+    del o
+    out = {"value": 1.0 / el, "unit": "steps/s (64 FP32 descent iterations per step)", "cores": cores, "kind": "port",
+           "sample": f"full step: one step of 64 FP32 descent iterations ({st.iterations} run) of the same "
+                     f"{len(tets)}-tet bar from rest on {cores} host threads in {el:.1f} s",
+           "descent_iterations_per_s": st.iterations / el}
+    if gpu_fixed_sps:
+        out["gpu_same_workload_steps_per_s"] = gpu_fixed_sps
+        out["gpu_over_cpu"] = gpu_fixed_sps * el
+    if relgrad:
+        p = O.default_params(iterations=40000, relgrad_tol=1e-4, grav=grav, dt=1.0 / 30.0, profile=prof)
+        o = O.Solver(rest, tets, mask, params=p, precision=64, material=material, threads=cores)
+        o.set_state(x=rest.astype(np.float64))
+        t0 = time.perf_counter()
+        st = o.step()
+        el = time.perf_counter() - t0
+        out["relgrad_full_step"] = {"steps_per_s": 1.0 / el, "iterations": st.iterations, "seconds": el,
+                                    "converged": st.converged, "precision": "f64"}
+    return out
 
 
 def cpu_baseline_batch(inst, iters, grav, sample=8):
@@ -110,7 +117,7 @@ def cpu_baseline_batch(inst, iters, grav, sample=8):
     each; the batch step time is extrapolated by tet count."""
     from oracle import pyoracle as O
 
-    cores = min(16, len(os.sched_getaffinity(0)))
+    cores = host_threads()
     t_sum, tets_sum = 0.0, 0
     for rest, tets, mask, x0, E, nu in inst[:sample]:
         p = O.default_params(iterations=iters, grav=grav, dt=1.0 / 30.0, profile=[(1, 0.96), (7, 0.991), (10, 0.9995)])
@@ -127,9 +134,9 @@ def cpu_baseline_batch(inst, iters, grav, sample=8):
                       f"({tets_sum} tets), scaled by tet count to the whole batch"}
 
 
-# the learned correction is smoothed by 100 averaging sweeps and tried at 2, 1.5 and 1 times its
-# length (the best of the three, if better than the plain guess, starts the descent)
-WARM_OPTIONS = dict(sweeps=140, betas=(1.0,))
+# the learned correction is smoothed by 20 averaging sweeps over the node graph and taken at its
+# full length; the gradient test then decides between it and the plain guess
+WARM_OPTIONS = dict(sweeps=20, betas=(1.0,))
 
 
 def impulse(rest, rng, k):
@@ -141,30 +148,76 @@ def impulse(rest, rng, k):
     return ext
 
 
-def train_warm_start(P, dims, params, steps, seed, epochs, capacity):
-    """The plain solver generates (state, converged step) pairs on a seeded trajectory (twist,
-    jitter, impulses) of the given bar; the network is trained on them on the GPU."""
-    nx, ny, nz = dims
-    rest, tets = P.mesh_bar(nx, ny, nz, 0.05)
-    mask = (rest[:, 0] == 0).astype(np.int32)
+def collect_trajectory(P, rest, tets, mask, x0, params, steps, seed, stride, capacity):
+    """The plain solver steps a seeded trajectory (impulses every 10th step) and keeps (features,
+    converged step) pairs of every stride-th free node per step.  Returns the solver."""
     s = P.Solver(rest, tets, mask, params)
-    s.set_positions(P.mesh_twist(rest, nx, ny, nz, 0.05, 45.0, 0.01, seed))
-    s.learn_collect(capacity)
+    s.set_positions(x0)
+    s.learn_collect(capacity, stride)
     rng = np.random.default_rng(seed)
-    t0 = time.perf_counter()
     for k in range(steps):
         s.step(impulse(rest, rng, k))
-    t1 = time.perf_counter()
+    return s
+
+
+def train_config4(P, params, c2_dims, epochs=30, seed=7):
+    """The warm start's training set and network (configuration 4): pairs generated by the plain
+    solver on seeded trajectories of configurations 1-3, none of them the evaluation trajectories
+    (those start from rest without jitter, and from a 90 degree twist with seed 11):
+      * the 64x64x256 bar from rest under gravity with 0.2 % jitter (seed 7), 30 steps, every 4th
+        free node per step;
+      * the 12x12x50 bar released from 90 degree twists with 1 % jitter (seeds 2 and 3), 60 steps
+        each, every 2nd free node;
+      * 6 bars of the batched configuration's distribution (seed 5) from rest, 40 steps each.
+    All with impulses on 20 random nodes every 10th step.  One network (Adam, 30 epochs) on the
+    pooled samples, evaluated on both the 64x64x256 and the 12x12x50 bar.  A second network trained
+    the same way on the 64x64x256 samples alone (the benchmark bar's own regime, same held-out
+    trajectory) is returned beside it: pooling the smaller bars' samples costs the large bar part
+    of its gain.  Returns (pooled network, bar network, info)."""
+    t0 = time.perf_counter()
+    rest, tets = P.mesh_bar(*c2_dims, 0.05)
+    mask = (rest[:, 0] == 0).astype(np.int32)
+    x0 = P.mesh_twist(rest, *c2_dims, 0.05, 0.0, 0.002, seed)
+    s = collect_trajectory(P, rest, tets, mask, x0, params, 30, seed, 4, 12_000_000)
+    counts = {"bar64x64x256": s.learn_samples()}
+    s.learn_fit_scales()
+    s.learn_init_model(seed)
+    loss_bar = s.learn_train(epochs, batch=8192, lr=2e-3, seed=seed)
+    model_bar = s.learn_get_model()
+
+    def pool(r, t, m, x, prm, steps, sd, stride):
+        q = collect_trajectory(P, r, t, m, x, prm, steps, sd, stride, 2_000_000)
+        f, tg = q.learn_get_samples()
+        q.close()
+        s.learn_add_samples(f, tg)
+        return len(f)
+
+    r1, t1 = P.mesh_bar(12, 12, 50, 0.05)
+    m1 = (r1[:, 0] == 0).astype(np.int32)
+    counts["bar12x12x50"] = sum(pool(r1, t1, m1, P.mesh_twist(r1, 12, 12, 50, 0.05, 90.0, 0.01, sd), params, 60, sd, 2)
+                                for sd in (2, 3))
+    n3 = 0
+    for i, (r, t, m, x, E, nu) in enumerate(batch_instances(P, count=6, seed=5)):
+        prm = P.default_params(**{k: getattr(params, k) for k in ("dt", "precision", "relgrad_tol", "max_iterations")},
+                               grav=tuple(params.grav), youngs=E, nu=nu,
+                               profile=[(params.profile_k[j], params.profile_rho[j]) for j in range(params.profile_n)])
+        n3 += pool(r, t, m, x, prm, 40, 20 + i, 1)
+    counts["batch_bars"] = n3
+    t1c = time.perf_counter()
     s.learn_fit_scales()
     s.learn_init_model(seed)
     loss = s.learn_train(epochs, batch=8192, lr=2e-3, seed=seed)
     t2 = time.perf_counter()
     model = s.learn_get_model()
-    info = {"trajectory": f"{nx}x{ny}x{nz} bar, 45 deg twist, 1% jitter, impulses, seed {seed}",
-            "steps": steps, "samples": s.learn_samples(), "epochs": epochs,
-            "loss_first_last": [float(loss[0]), float(loss[-1])], "collect_s": t1 - t0, "train_s": t2 - t1}
+    info = {"samples": int(s.learn_samples()), "samples_by_configuration": counts, "epochs": epochs,
+            "loss_first_last": [float(loss[0]), float(loss[-1])],
+            "bar_network_loss_first_last": [float(loss_bar[0]), float(loss_bar[-1])],
+            "collect_and_bar_network_s": t1c - t0, "train_s": t2 - t1c,
+            "trajectories": "64x64x256 from rest, 0.2% jitter, seed 7, 30 steps, every 4th node; 12x12x50 from "
+                            "90 deg twists, 1% jitter, seeds 2 and 3, 60 steps, every 2nd node; 6 batch bars "
+                            "(seed 5) from rest, 40 steps; impulses every 10th step"}
     s.close()
-    return model, info
+    return model, model_bar, info
 
 
 def timed_steps(P, rest, tets, mask, x0, params, warmup, steps, model=None):
@@ -187,14 +240,18 @@ def timed_steps(P, rest, tets, mask, x0, params, warmup, steps, model=None):
 
 
 def warm_start_report(P, args, params, rest, tets, mask, x0, dims, plain_sps, plain_its, other=None):
-    """Learned warm start vs plain: on the benchmark bar (trained on another trajectory of the same
-    bar) and on the 12x12x50 bar (43,200 tets).  other = (params, plain result) of the other
-    arithmetic (mixed / FP64 throughout): the same trained network there too."""
+    """Learned warm start vs plain with one network trained per configuration 4 (train_config4):
+    on the benchmark bar (the timed steps' own protocol: warm-up, then the timed steps) and on the
+    12x12x50 bar released from a 90 degree twist (seed 11; 1 warm-up step, 30 timed).  other =
+    (params, plain result) of the other arithmetic (mixed / FP64 throughout): the same network."""
     out = {}
-    model, info = train_warm_start(P, dims, params, args.train_steps, 7, 30, 14_000_000)
+    model, model_bar, info = train_config4(P, params, dims)
+    out["training"] = info
     lw = timed_steps(P, rest, tets, mask, x0, params, args.warmup, args.steps, model)
-    out["bench_bar"] = {"training": info, "plain_steps_per_s": plain_sps, "plain_iterations_per_step": plain_its,
+    out["bench_bar"] = {"plain_steps_per_s": plain_sps, "plain_iterations_per_step": plain_its,
                         "learned": lw, "speedup": lw["steps_per_s"] / plain_sps}
+    lb = timed_steps(P, rest, tets, mask, x0, params, args.warmup, args.steps, model_bar)
+    out["bench_bar_bar_network"] = {"learned": lb, "speedup": lb["steps_per_s"] / plain_sps}
     if other is not None:
         op, oplain = other
         lo = timed_steps(P, rest, tets, mask, x0, op, args.warmup, args.steps, model)
@@ -204,11 +261,9 @@ def warm_start_report(P, args, params, rest, tets, mask, x0, dims, plain_sps, pl
     r1, t1 = P.mesh_bar(*d1, 0.05)
     m1 = (r1[:, 0] == 0).astype(np.int32)
     x1 = P.mesh_twist(r1, *d1, 0.05, 90.0, 0.01, 11)
-    model1, info1 = train_warm_start(P, d1, params, 120, 1, 30, 2_000_000)
     p1 = timed_steps(P, r1, t1, m1, x1, params, 1, 30)
-    l1 = timed_steps(P, r1, t1, m1, x1, params, 1, 30, model1)
-    out["bar_12x12x50"] = {"tets": len(t1), "training": info1, "plain": p1, "learned": l1,
-                           "speedup": l1["steps_per_s"] / p1["steps_per_s"]}
+    l1 = timed_steps(P, r1, t1, m1, x1, params, 1, 30, model)
+    out["bar_12x12x50"] = {"tets": len(t1), "plain": p1, "learned": l1, "speedup": l1["steps_per_s"] / p1["steps_per_s"]}
     out["options"] = {"smoothing_sweeps": WARM_OPTIONS["sweeps"], "steps_tried": list(WARM_OPTIONS["betas"])}
     return out
 
@@ -346,9 +401,10 @@ def main():
                          "(0: FP64 throughout)")
     ap.add_argument("--max-iterations", type=int, default=40000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-relgrad", action="store_true",
+                    help="also time one full FP64 rel-grad step of the CPU path (minutes)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
     ap.add_argument("--no-implicit", action="store_true", help="skip the backward-Euler report")
-    ap.add_argument("--train-steps", type=int, default=12, help="training-trajectory steps on the benchmark bar")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) for counter calibration")
     args = ap.parse_args()
@@ -469,6 +525,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": MEASURED_TRAFFIC.get((dom, rsz)) if args.workload == "bar64x64x256" else None,
+                     "traffic_source": (TRAFFIC_PROFILE + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; not this run)")
+                     if args.workload == "bar64x64x256" and (dom, rsz) in MEASURED_TRAFFIC else None,
                      "kernel": dom,
                      "algorithmic_bytes_per_launch": alg[dom]},
         "learned_warm_start": None,
@@ -493,6 +551,8 @@ def main():
         out["learned_steps_per_s"] = lw["bench_bar"]["learned"]["steps_per_s"]
         out["learned_speedup"] = lw["bench_bar"]["speedup"]
         out["learned_speedup_43k"] = lw["bar_12x12x50"]["speedup"]
+        out["learned_speedup_bar_network"] = lw["bench_bar_bar_network"]["speedup"]
+        out["learned_guess_used_fraction"] = lw["bench_bar"]["learned"]["guess_used_fraction"]
     if not batch and args.workload == "bar64x64x256":
         out["fixed_64_iterations_f32"] = fixed_iteration_report(P, rest, tets, mask, grav)
     if not args.no_implicit and not batch:
@@ -503,7 +563,9 @@ def main():
     if not args.no_cpu_baseline and batch:
         out["cpu_baseline"] = cpu_baseline_batch(inst, fixed_iters, grav)
     elif not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(rest, tets, mask, material, iters / args.steps, grav)
+        fixed = out.get("fixed_64_iterations_f32", {}).get("neo_hookean" if material == 0 else "stretch", {})
+        out["cpu_baseline"] = cpu_baseline(rest, tets, mask, material, grav, fixed.get("steps_per_s"),
+                                           relgrad=args.cpu_relgrad)
     print(json.dumps(out))
 
 

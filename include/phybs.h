@@ -217,6 +217,11 @@ int pbs_implicit_step(pbs_solver *s, const pbs_implicit_params *p, const float *
 /* K(x) = -df/dx at the current positions in the reference's CSR layout (computeGlobalK,
  * CUDAInterface.cu:234-271; pattern = pbs_csr_pattern's rowptr/col, nnz values). */
 int pbs_stiffness_matrix(pbs_solver *s, int64_t nnz, double *csr_val);
+/* IsotropicMaterial::computeStiffnessMatrixFromPos(tetID, pos) (IsotropicMaterial.cpp:503-552) for
+ * num tets at once: K (144 FP32 values per listed tet, row-major 12 x 12, row 3a + d and column
+ * 3b + e for the tet's corners a, b in its own corner order and axes d, e) = -df/dx of the tet's
+ * elastic forces at positions pos (3N AoS).  Single-mesh handles only. */
+int pbs_tet_stiffness(pbs_solver *s, const float *pos, int num, const int *tet_ids, float *K);
 
 /* ---- projective dynamics (ProjDynamic::doProjDynamics, ProjDynamic.cpp:442-479) ---- */
 typedef struct pbs_pd_params {
@@ -256,6 +261,14 @@ int pbs_mesh_twist(int nx, int ny, int nz, float edge, double twist_deg, double 
  * multiplied by scale (the reference uses 0.1, TetMesh.cpp:38).  faces may be NULL. */
 int pbs_tetgen_count(const char *prefix, int *num_nodes, int *num_tets, int *num_faces);
 int pbs_tetgen_load(const char *prefix, float scale, float *nodes, int *tets, int *faces);
+/* One file at a time, as TetMesh::initNodesFromFile / initTetsFromFile / initFacesFromFile
+ * (TetMesh.h:20-22; TetMesh.cpp:14-56, 96-125, 127-151) read them: the count is returned in
+ * *num_*; the array (3 floats per node, times scale; 4 or 3 ints per tet or face) is filled when
+ * non-NULL.  The reference's hard-coded constraint rule is not applied here (phybs.hpp's TetMeshT
+ * applies it). */
+int pbs_tetgen_read_nodes(const char *path, float scale, int *num_nodes, float *nodes);
+int pbs_tetgen_read_tets(const char *path, int *num_tets, int *tets);
+int pbs_tetgen_read_faces(const char *path, int *num_faces, int *faces);
 /* .node export / import (TetMesh.cpp:558-570 / :70-94, no scaling on import). */
 int pbs_node_write(const char *path, int num_nodes, const float *nodes);
 int pbs_node_read(const char *path, int num_nodes, float *nodes);
@@ -309,8 +322,15 @@ int pbs_learn_set_options(pbs_solver *s, int sweeps, int nbeta, const float *bet
  * capacity samples (0 stops and frees); collection stops when the next step would not fit.
  * Samples are stored step after step, the free nodes of a step in the caller's node order. */
 int pbs_learn_collect(pbs_solver *s, int64_t capacity);
+/* The same, keeping from the k-th collected step only the free nodes whose rank r among the free
+ * nodes (caller's order) has r % stride == k % stride: every node once per stride steps. */
+int pbs_learn_collect_every(pbs_solver *s, int64_t capacity, int stride);
 int64_t pbs_learn_samples(pbs_solver *s);
 int pbs_learn_get_samples(pbs_solver *s, int64_t start, int64_t n, float *features, float *targets);
+/* Appends n samples (18 features, 3 targets each) to the collected ones, e.g. those another
+ * handle collected on another mesh, so that one network trains on several configurations;
+ * fails if they do not fit the capacity given to pbs_learn_collect*. */
+int pbs_learn_add_samples(pbs_solver *s, int64_t n, const float *features, const float *targets);
 /* Scales = RMS of each feature group and of the targets over the collected samples. */
 int pbs_learn_fit_scales(pbs_solver *s);
 /* Adam (lr, beta 0.9/0.999) on the mean squared error of the normalised outputs, mini-batches of

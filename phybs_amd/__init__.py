@@ -128,6 +128,10 @@ SIGNATURES = {
                                  C.c_int, _P]),
     "pbs_tetgen_count": (C.c_int, [C.c_char_p, _P, _P, _P]),
     "pbs_tetgen_load": (C.c_int, [C.c_char_p, C.c_float, _P, _P, _P]),
+    "pbs_tetgen_read_nodes": (C.c_int, [C.c_char_p, C.c_float, _P, _P]),
+    "pbs_tetgen_read_tets": (C.c_int, [C.c_char_p, _P, _P]),
+    "pbs_tetgen_read_faces": (C.c_int, [C.c_char_p, _P, _P]),
+    "pbs_tet_stiffness": (C.c_int, [_P, _P, C.c_int, _P, _P]),
     "pbs_node_write": (C.c_int, [C.c_char_p, C.c_int, _P]),
     "pbs_node_read": (C.c_int, [C.c_char_p, C.c_int, _P]),
     "pbs_csr_pattern": (C.c_int64, [C.c_int, C.c_int, _P, _P, _P, _P, _P]),
@@ -147,6 +151,8 @@ SIGNATURES = {
     "pbs_learn_enable": (C.c_int, [_P, C.c_int]),
     "pbs_learn_set_options": (C.c_int, [_P, C.c_int, C.c_int, _P]),
     "pbs_learn_collect": (C.c_int, [_P, C.c_int64]),
+    "pbs_learn_collect_every": (C.c_int, [_P, C.c_int64, C.c_int]),
+    "pbs_learn_add_samples": (C.c_int, [_P, C.c_int64, _P, _P]),
     "pbs_learn_samples": (C.c_int64, [_P]),
     "pbs_learn_get_samples": (C.c_int, [_P, C.c_int64, C.c_int64, _P, _P]),
     "pbs_learn_fit_scales": (C.c_int, [_P]),
@@ -434,6 +440,14 @@ class Solver:
         _check(lib().pbs_stiffness_matrix(self.h, len(col), _p(val)))
         return rowptr, col, val
 
+    def tet_stiffness(self, pos, tet_ids):
+        """Per-tet 12 x 12 stiffness K = -df/dx of the listed tets at pos (FP32)."""
+        pos = _f32(pos)
+        ids = np.ascontiguousarray(tet_ids, np.int32).reshape(-1)
+        K = np.zeros((len(ids), 12, 12), np.float32)
+        _check(lib().pbs_tet_stiffness(self.h, _p(pos), len(ids), _p(ids), _p(K)))
+        return K
+
     def do_descent_opt(self, h, iterations, ext):
         e = _f32(ext)
         out = np.zeros((self.nn, 3), np.float32)
@@ -504,8 +518,16 @@ class Solver:
         b = np.ascontiguousarray(betas, np.float32)
         _check(lib().pbs_learn_set_options(self.h, int(sweeps), len(b), _p(b)))
 
-    def learn_collect(self, capacity):
-        _check(lib().pbs_learn_collect(self.h, int(capacity)))
+    def learn_collect(self, capacity, stride=1):
+        """Collect (features, converged step) samples from the following steps; with stride > 1 each
+        step keeps every stride-th free node (a different residue class each step)."""
+        _check(lib().pbs_learn_collect_every(self.h, int(capacity), int(stride)))
+
+    def learn_add_samples(self, features, targets):
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, 18)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        assert len(f) == len(t)
+        _check(lib().pbs_learn_add_samples(self.h, len(f), _p(f), _p(t)))
 
     def learn_samples(self):
         n = lib().pbs_learn_samples(self.h)

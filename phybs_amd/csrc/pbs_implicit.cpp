@@ -458,6 +458,38 @@ template <typename R> int stiffness_impl(pbs_solver *s, int64_t nnz, double *csr
   return PBS_OK;
 }
 
+template <typename R> int tet_stiffness_impl(pbs_solver *s, const float *pos, int num, const int *ids, float *K) {
+  DevMesh<R> &m = mesh<R>(s);
+  std::vector<int> sid(std::max(num, 1));
+  for (int k = 0; k < num; ++k) {
+    if (ids[k] < 0 || ids[k] >= s->nt) return set_error(PBS_ERR_ARG, "pbs_tet_stiffness: tet id out of range");
+    sid[k] = s->plan.itperm[ids[k]];
+  }
+  int *dids = nullptr;
+  float *dK = nullptr;
+  HIPCHK(hipMalloc(&dids, sizeof(int) * sid.size()));
+  hipError_t e = hipMalloc(&dK, sizeof(float) * 144 * (size_t)std::max(num, 1));
+  if (e != hipSuccess) {
+    (void)hipFree(dids);
+    return set_error(PBS_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  e = hipMemcpyAsync(s->stage, pos, sizeof(float) * 3 * (size_t)s->nn, hipMemcpyHostToDevice, s->st);
+  if (e == hipSuccess) {
+    Launch<R>::aos_to_planes(s->stage, m.ring.b[0], m.perm, s->nn, s->st);
+    e = hipMemcpyAsync(dids, sid.data(), sizeof(int) * num, hipMemcpyHostToDevice, s->st);
+  }
+  if (e == hipSuccess) {
+    IeLaunch<R>::tet_stiffness(m, s->dp, s->td.inst_mu, s->td.inst_lam, m.ring.b[0], num, dids, dK, s->st);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(K, dK, sizeof(float) * 144 * (size_t)num, hipMemcpyDeviceToHost, s->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+  (void)hipFree(dids);
+  (void)hipFree(dK);
+  if (e != hipSuccess) return set_error(PBS_ERR_HIP, std::string("pbs_tet_stiffness: ") + hipGetErrorString(e));
+  return PBS_OK;
+}
+
 }  // namespace
 
 extern "C" int pbs_implicit_default_params(pbs_implicit_params *p) {
@@ -478,6 +510,14 @@ extern "C" int pbs_implicit_step(pbs_solver *s, const pbs_implicit_params *ip, c
   if (rc) return rc;
   return s->p.precision == PBS_FP64 ? implicit_step_impl<double>(s, ip, ext, out, stats)
                                     : implicit_step_impl<float>(s, ip, ext, out, stats);
+}
+
+extern "C" int pbs_tet_stiffness(pbs_solver *s, const float *pos, int num, const int *tet_ids, float *K) {
+  if (!s || !pos || num < 0 || (num > 0 && (!tet_ids || !K))) return set_error(PBS_ERR_ARG, "pbs_tet_stiffness: bad arguments");
+  if (s->ninst != 1) return set_error(PBS_ERR_STATE, "pbs_tet_stiffness: single-mesh handles only");
+  if (num == 0) return PBS_OK;
+  return s->p.precision == PBS_FP64 ? tet_stiffness_impl<double>(s, pos, num, tet_ids, K)
+                                    : tet_stiffness_impl<float>(s, pos, num, tet_ids, K);
 }
 
 extern "C" int pbs_stiffness_matrix(pbs_solver *s, int64_t nnz, double *csr_val) {

@@ -46,6 +46,9 @@ template <typename R> struct IeLaunch {
   static void assemble(const DevMesh<R> &m, const TileDev &td, const DevParams &p, const IeDev &d, const IeAsm &a,
                        int max_width, hipStream_t st);
   static void start(const IeDev &d, hipStream_t st);
+  // per-tet 12 x 12 stiffness of the solver tets ids[0 .. num) at positions x (planes), FP32 out
+  static void tet_stiffness(const DevMesh<R> &m, const DevParams &p, const float *inst_mu, const float *inst_lam,
+                            const R *x, int num, const int *ids, float *out, hipStream_t st);
   // one CG iteration = spmv, alpha, update, beta (+ the search-direction update), each a kernel
   static void spmv(const IeDev &d, hipStream_t st);
   static void alpha(const IeDev &d, hipStream_t st);

@@ -109,8 +109,8 @@ __device__ __forceinline__ int find_col(const int *__restrict__ bcol, int g0, in
 // derivative the reference evaluates through the SVD and Xu et al. 2015's 2x2 solves
 // (compute_dfdx, CUDAInterface.cu:101-219).  The elastic force of the corner comes from the same
 // per-tet routine as the descent solver's.
-// LACC: the row is accumulated in LDS (one 64-row slice per workgroup, each lane its own column
-// of slots, so no lane touches another's) and written to the matrix once; otherwise it is
+// LACC: the row (`row`) is accumulated in LDS (one 64-row slice per workgroup, each lane its own
+// column of slots, so no lane touches another's) and written to the matrix once; otherwise it is
 // accumulated in place in the matrix.
 // ---------------------------------------------------------------------------------------------
 template <typename R, int MAT, bool LACC>
@@ -129,11 +129,11 @@ __global__ void __launch_bounds__(kBlock)
   const int g0 = d.slice_off[n >> 6];
   const int width = d.slice_off[(n >> 6) + 1] - g0;
   const int len = d.row_len[n];
-  R *__restrict__ acc = LACC ? (R *)ie_lds : A;
+  R *__restrict__ row = LACC ? (R *)ie_lds : A;
   auto ix = [&](int j, int q) -> size_t { return LACC ? ((size_t)j * 9 + q) * 64 + lane : val_idx(g0 + j, q, lane); };
   for (int j = 0; j < width; ++j)
 #pragma unroll
-    for (int q = 0; q < 9; ++q) acc[ix(j, q)] = (R)0;
+    for (int q = 0; q < 9; ++q) row[ix(j, q)] = (R)0;
   const R mu = (R)inst_mu[0], lam = (R)inst_lam[0];
   R f[3] = {(R)0, (R)0, (R)0};
   for (int e = inc_off[n]; e < inc_off[n + 1]; ++e) {
@@ -237,7 +237,7 @@ __global__ void __launch_bounds__(kBlock)
             for (int q = 0; q < 3; ++q) rot += ai[q][dd] * aj[q][ee] * aw[q];
             kv -= mu2 * rot;
           }
-          acc[ix(slot_j, 3 * dd + ee)] += V * kv;
+          row[ix(slot_j, 3 * dd + ee)] += V * kv;
         }
     }
   }
@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(kBlock)
     if (LACC)
       for (int j = 0; j < width; ++j)
 #pragma unroll
-        for (int q = 0; q < 9; ++q) A[val_idx(g0 + j, q, lane)] = acc[ix(j, q)];
+        for (int q = 0; q < 9; ++q) A[val_idx(g0 + j, q, lane)] = row[ix(j, q)];
   };
   if (a.raw) {
     flush();
@@ -257,7 +257,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
       const size_t i = ix(j, q);
-      acc[i] = acc[i] * s1 * s2;
+      row[i] = row[i] * s1 * s2;
     }
   const R ms = (R)mass[n] * ((R)1 + (R)a.alpha * (R)a.h);
   const int ds = d.diag_slot[n];
@@ -265,8 +265,8 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const size_t i = ix(dj, 4 * k);
-    acc[i] += ms;
-    d.dinv[3 * (size_t)n + k] = 1.0 / (double)acc[i];
+    row[i] += ms;
+    d.dinv[3 * (size_t)n + k] = 1.0 / (double)row[i];
   }
   flush();
   // right-hand side h f + m v (setRHSofLinearSystem :418-442) with the forces of
@@ -286,6 +286,103 @@ __global__ void __launch_bounds__(kBlock)
     d.b[3 * (size_t)n + k] = (double)(fk * h + (R)mass[n] * vk);
     d.x[3 * (size_t)n + k] = a.warm ? (double)vk : 0.0;
   }
+}
+
+// Per-tet stiffness K = -df/dx (12 x 12, rows 3 a + d, columns 3 b + e over the tet's corners a,
+// b and axes d, e) at the positions in planes x: the blocks of the closed form the assembly above
+// accumulates, one thread per requested tet (IsotropicMaterial::computeStiffnessMatrixFromPos,
+// IsotropicMaterial.cpp:503-552).  ids: solver tet ids; out: 144 FP32 values per requested tet.
+template <typename R, int MAT>
+__global__ void k_tet_stiffness(DevParams p, int num, const int *__restrict__ ids, const int4 *__restrict__ tets,
+                                const float *__restrict__ dmi, const float *__restrict__ vol, const R *__restrict__ x,
+                                const float *__restrict__ inst_mu, const float *__restrict__ inst_lam,
+                                float *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= num) return;
+  const int t = ids[k], N = p.nn, T = p.nt;
+  const int4 q4 = tets[t];
+  const int qn[4] = {q4.x, q4.y, q4.z, q4.w};
+  R Dmi[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Dmi[i] = (R)dmi[(size_t)i * T + t];
+  const R V = (R)vol[t];
+  const R mu = (R)inst_mu[0], lam = (R)inst_lam[0];
+  R X[4][3];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) X[c][d] = x[(size_t)d * N + qn[c]];
+  R Ds[9], F[9], C[9];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    Ds[3 * d + 0] = X[1][d] - X[0][d];
+    Ds[3 * d + 1] = X[2][d] - X[0][d];
+    Ds[3 * d + 2] = X[3][d] - X[0][d];
+  }
+  mul3(Ds, Dmi, F);
+  const R J = adj3(F, C);
+  const R iJ = (R)1 / J;
+  R c[4][3], g[4][3];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    shape_vec(Dmi, m, c[m]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) g[m][d] = (c[m][0] * C[d] + c[m][1] * C[3 + d] + c[m][2] * C[6 + d]) * iJ;
+  }
+  R k1, k2, U[9] = {}, Vv[9] = {}, aw[3] = {};
+  const int kk[3] = {0, 0, 1}, ll[3] = {1, 2, 2};
+  if (MAT == 0) {
+    const R lj = dlog(J);
+    k1 = mu - lam * lj;
+    k2 = lam;
+  } else {
+    R S[3];
+    svd3(F, U, S, Vv);
+    k1 = -lam * (J - (R)1) * J;
+    k2 = lam * ((R)2 * J - (R)1) * J;
+    const R lim = (sizeof(R) == 4 ? (R)1e-6 : (R)1e-12);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const R den = S[kk[q]] + S[ll[q]];
+      aw[q] = (R)1 / (den > lim ? den : lim);
+    }
+  }
+  const R mu2 = (MAT == 0) ? mu : (R)2 * mu;
+  // a[m][q][d] = (a^d_kl . c_m) for the pairs kl = 01, 02, 12 (stretch material)
+  R a[4][3][3] = {};
+  if (MAT != 0)
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          R acc = (R)0;
+#pragma unroll
+          for (int b = 0; b < 3; ++b)
+            acc += (U[3 * d + kk[q]] * Vv[3 * b + ll[q]] - U[3 * d + ll[q]] * Vv[3 * b + kk[q]]) * c[m][b];
+          a[m][q][d] = acc;
+        }
+  float *o = out + 144 * (size_t)k;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const R cc = c[i][0] * c[m][0] + c[i][1] * c[m][1] + c[i][2] * c[m][2];
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          R kv = (d == e ? mu2 * cc : (R)0) + k1 * g[m][d] * g[i][e] + k2 * g[i][d] * g[m][e];
+          if (MAT != 0) {
+            R rot = (R)0;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) rot += a[i][q][d] * a[m][q][e] * aw[q];
+            kv -= mu2 * rot;
+          }
+          o[12 * (3 * i + d) + 3 * m + e] = (float)(V * kv);
+        }
+    }
 }
 
 // r = b - A x0, z = D^-1 r, p = z; partials r.z, r.r, b.D^-1.b.  The stopping test uses the
@@ -629,6 +726,18 @@ inline int nb_of(int n) { return (n + kBlock - 1) / kBlock; }
 
 }  // namespace
 
+template <typename R>
+void IeLaunch<R>::tet_stiffness(const DevMesh<R> &m, const DevParams &p, const float *inst_mu, const float *inst_lam,
+                                const R *x, int num, const int *ids, float *out, hipStream_t st) {
+  if (num <= 0) return;
+  const int blocks = (num + 63) / 64;
+  if (p.material == 0)
+    hipLaunchKernelGGL((k_tet_stiffness<R, 0>), dim3(blocks), dim3(64), 0, st, p, num, ids, m.tets, m.dmi, m.vol, x,
+                       inst_mu, inst_lam, out);
+  else
+    hipLaunchKernelGGL((k_tet_stiffness<R, 1>), dim3(blocks), dim3(64), 0, st, p, num, ids, m.tets, m.dmi, m.vol, x,
+                       inst_mu, inst_lam, out);
+}
 template <typename R>
 void IeLaunch<R>::assemble(const DevMesh<R> &m, const TileDev &td, const DevParams &p, const IeDev &d,
                            const IeAsm &a, int max_width, hipStream_t st) {

@@ -150,9 +150,10 @@ template <typename R> struct LearnLaunch {
   static void select(const DevMesh<R> &m, const TileDev &td, const DevParams &p, const int *adj_off, const int *adj,
                      const int *sell_off, const int *sell_col, int iters, const float *betas, int nbeta, float *sm,
                      hipStream_t st);
-  // appends (features, x_new - y) of free nodes to the sample buffers
+  // appends (features, x_new - y) of the free nodes of rank r with r % stride == phase to the
+  // sample buffers (rank r at base + r / stride)
   static void collect(const DevMesh<R> &m, const DevParams &p, const float *feat, const R *y, const int *free_rank,
-                      long base, float *ds_x, float *ds_y, hipStream_t st);
+                      long base, int stride, int phase, float *ds_x, float *ds_y, hipStream_t st);
 };
 void mlp_arrange(const float *params, unsigned short *wA, hipStream_t st);
 void mlp_forward(const MlpDev &net, const float *feat, int n, float *out, hipStream_t st);

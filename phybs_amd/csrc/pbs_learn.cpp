@@ -183,13 +183,17 @@ template <typename R> int learn_before_descent(pbs_solver *s) {
 template <typename R> int learn_after_step(pbs_solver *s) {
   LearnState *L = s->learn;
   if (!L || !L->collecting) return PBS_OK;
-  if (L->count + L->nfree > L->cap) {
+  const int phase = (int)(L->steps_seen % L->stride);
+  const long take = L->nfree > phase ? (L->nfree - phase + L->stride - 1) / L->stride : 0;
+  if (L->count + take > L->cap) {
     L->collecting = false;  // full
     return PBS_OK;
   }
-  LearnLaunch<R>::collect(mesh<R>(s), s->dp, L->feat, (const R *)L->y, L->free_rank, L->count, L->ds_x, L->ds_y, s->st);
+  LearnLaunch<R>::collect(mesh<R>(s), s->dp, L->feat, (const R *)L->y, L->free_rank, L->count, L->stride, phase,
+                          L->ds_x, L->ds_y, s->st);
   HIPCHK(hipGetLastError());
-  L->count += L->nfree;
+  L->count += take;
+  L->steps_seen++;
   return PBS_OK;
 }
 template int learn_before_descent<float>(pbs_solver *);
@@ -264,13 +268,18 @@ extern "C" int pbs_learn_set_options(pbs_solver *s, int sweeps, int nbeta, const
   return PBS_OK;
 }
 
-extern "C" int pbs_learn_collect(pbs_solver *s, int64_t capacity) {
+extern "C" int pbs_learn_collect(pbs_solver *s, int64_t capacity) { return pbs_learn_collect_every(s, capacity, 1); }
+
+extern "C" int pbs_learn_collect_every(pbs_solver *s, int64_t capacity, int stride) {
   LEARN_GET(s, L);
   if (capacity < 0) return set_error(PBS_ERR_ARG, "capacity must be >= 0");
+  if (stride < 1) return set_error(PBS_ERR_ARG, "stride must be >= 1");
   lfree(L, L->ds_x);
   lfree(L, L->ds_y);
   L->ds_x = L->ds_y = nullptr;
   L->cap = L->count = 0;
+  L->stride = stride;
+  L->steps_seen = 0;
   L->collecting = false;
   if (capacity == 0) return PBS_OK;
   int rc = lalloc(L, &L->ds_x, 18 * (size_t)capacity);
@@ -294,6 +303,19 @@ extern "C" int pbs_learn_get_samples(pbs_solver *s, int64_t start, int64_t n, fl
   if (targets && n)
     HIPCHK(hipMemcpyAsync(targets, L->ds_y + 3 * start, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
+  return PBS_OK;
+}
+
+extern "C" int pbs_learn_add_samples(pbs_solver *s, int64_t n, const float *features, const float *targets) {
+  LEARN_GET(s, L);
+  if (n < 0 || (n > 0 && (!features || !targets))) return set_error(PBS_ERR_ARG, "pbs_learn_add_samples: bad arguments");
+  if (L->count + n > L->cap) return set_error(PBS_ERR_ARG, "pbs_learn_add_samples: sample buffer too small");
+  if (n) {
+    HIPCHK(hipMemcpyAsync(L->ds_x + 18 * L->count, features, sizeof(float) * 18 * n, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(L->ds_y + 3 * L->count, targets, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+  }
+  L->count += n;
   return PBS_OK;
 }
 

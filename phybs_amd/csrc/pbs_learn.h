@@ -29,6 +29,8 @@ struct LearnState {
   // collected samples
   bool collecting = false;
   long cap = 0, count = 0;
+  int stride = 1;       // each step keeps the free nodes of rank r with r % stride == phase
+  long steps_seen = 0;  // collected steps (phase = steps_seen % stride)
   float *ds_x = nullptr, *ds_y = nullptr;
   // training scratch
   float *grad_part = nullptr, *loss_part = nullptr;

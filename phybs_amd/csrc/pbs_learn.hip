@@ -341,14 +341,14 @@ __global__ void __launch_bounds__(1024) k_ws_stage(Ctrl *__restrict__ ctrl, cons
 // Appends (features, x_new - y) of free nodes: sample base + free_rank[n].
 template <typename R>
 __global__ void k_collect(DevParams p, const R *__restrict__ xnew, const float *__restrict__ feat, const R *__restrict__ y,
-                          const int *__restrict__ free_rank, long base, float *__restrict__ ds_x,
-                          float *__restrict__ ds_y) {
+                          const int *__restrict__ free_rank, long base, int stride, int phase,
+                          float *__restrict__ ds_x, float *__restrict__ ds_y) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   const int N = p.nn;
   if (n >= N) return;
   const int r = free_rank[n];
-  if (r < 0) return;
-  const size_t j = (size_t)(base + r);
+  if (r < 0 || r % stride != phase) return;
+  const size_t j = (size_t)(base + r / stride);
 #pragma unroll
   for (int k = 0; k < 18; ++k) ds_x[j * 18 + k] = feat[(size_t)n * 18 + k];
 #pragma unroll
@@ -559,9 +559,10 @@ void LearnLaunch<R>::select(const DevMesh<R> &m, const TileDev &td, const DevPar
 }
 template <typename R>
 void LearnLaunch<R>::collect(const DevMesh<R> &m, const DevParams &p, const float *feat, const R *y,
-                             const int *free_rank, long base, float *ds_x, float *ds_y, hipStream_t st) {
+                             const int *free_rank, long base, int stride, int phase, float *ds_x, float *ds_y,
+                             hipStream_t st) {
   hipLaunchKernelGGL((k_collect<R>), dim3(nb(p.nn, 256)), dim3(256), 0, st, p, (const R *)m.xpos, feat, y, free_rank,
-                     base, ds_x, ds_y);
+                     base, stride, phase, ds_x, ds_y);
 }
 template struct LearnLaunch<float>;
 template struct LearnLaunch<double>;

@@ -161,6 +161,37 @@ extern "C" int pbs_tetgen_load(const char *prefix, float scale, float *nodes, in
   return PBS_OK;
 }
 
+// One TetGen file at a time, as TetMesh::initNodesFromFile / initTetsFromFile / initFacesFromFile
+// (TetMesh.cpp:14-56, 96-125, 127-151) read them: with a NULL output only the count is returned.
+template <typename T>
+static int read_one(const char *path, int ncol, int *count, T *out, double scale, const char *what) {
+  if (!path || !count) return set_error(PBS_ERR_ARG, std::string(what) + ": null argument");
+  std::ifstream f(path);
+  long c;
+  if (!f || !read_header(f, c) || c < 0 || c > 2147483647L) return set_error(PBS_ERR_IO, std::string("cannot read ") + path);
+  *count = (int)c;
+  if (!out) return PBS_OK;
+  std::vector<double> rows;
+  if (!read_rows(f, c, ncol, rows)) return set_error(PBS_ERR_IO, std::string("cannot parse ") + path);
+  for (long i = 0; i < (long)ncol * c; ++i) out[i] = (T)(rows[i] * scale);
+  return PBS_OK;
+}
+extern "C" int pbs_tetgen_read_nodes(const char *path, float scale, int *num_nodes, float *nodes) {
+  int rc = read_one<float>(path, 3, num_nodes, nullptr, 1.0, "pbs_tetgen_read_nodes");
+  if (rc || !nodes) return rc;
+  // positions are scaled in FP32 after parsing, as pbs_tetgen_load does
+  rc = read_one<float>(path, 3, num_nodes, nodes, 1.0, "pbs_tetgen_read_nodes");
+  if (rc) return rc;
+  for (long i = 0; i < 3L * *num_nodes; ++i) nodes[i] = nodes[i] * scale;
+  return PBS_OK;
+}
+extern "C" int pbs_tetgen_read_tets(const char *path, int *num_tets, int *tets) {
+  return read_one<int>(path, 4, num_tets, tets, 1.0, "pbs_tetgen_read_tets");
+}
+extern "C" int pbs_tetgen_read_faces(const char *path, int *num_faces, int *faces) {
+  return read_one<int>(path, 3, num_faces, faces, 1.0, "pbs_tetgen_read_faces");
+}
+
 extern "C" int pbs_node_write(const char *path, int nn, const float *nodes) {
   if (!path || !nodes || nn < 0) return set_error(PBS_ERR_ARG, "pbs_node_write: bad arguments");
   FILE *f = std::fopen(path, "w");

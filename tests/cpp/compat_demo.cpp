@@ -7,11 +7,22 @@
 //   compat_demo timeint <steps>  TimeIntegrationT (no GPU): the step of each constructor, then
 //                                 <steps> explicit-Euler steps of the twisted bar under a constant
 //                                 force, and whether BackEuler throws
-// Output: one line per node, "x y z" with 9 significant digits.
+//   compat_demo material <TetGen prefix> <out file>
+//                                 TetMeshT loads the mesh file by file; IsotropicNeohookeanMaterialT
+//                                 evaluates forces, energies, the global and two per-tet stiffness
+//                                 matrices and the principal-stretch derivatives at a perturbed
+//                                 state; the arrays go to <out file> (int32 tag, int32 count, data)
+//   compat_demo descent_stretch <steps>
+//                                 DescentOptimizeT over TetMeshT-shaped data with the stretch
+//                                 material object: doDescentOpt(96) per step
+// Output (other than material): one line per node, "x y z" with 9 significant digits.
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "phybs.hpp"
@@ -32,7 +43,71 @@ struct Mat {
     m.v.assign((size_t)rows * cols, T(0));
     return m;
   }
+  static Mat Zero(int n) { return Zero(n, 1); }
 };
+typedef phybs::TetMeshT<Mat<float>, Mat<int>, Mat<float>> DemoTetMesh;
+
+// tag 0: float32 array, tag 1: int32 array
+template <class T>
+void put(FILE *f, const T *p, size_t n) {
+  const int tag = std::is_same<T, float>::value ? 0 : 1, cnt = (int)n;
+  std::fwrite(&tag, 4, 1, f);
+  std::fwrite(&cnt, 4, 1, f);
+  std::fwrite(p, sizeof(T), n, f);
+}
+
+int material_mode(const char *prefix, const char *out) {
+  auto mesh = std::make_shared<DemoTetMesh>();
+  mesh->setTetMaterial(1e5f, 0.45f, 1000.0f);
+  mesh->initNodesFromFile(std::string(prefix) + ".node");
+  mesh->initTetsFromFile(std::string(prefix) + ".ele");
+  mesh->initFacesFromFile(std::string(prefix) + ".face");
+  const int nn = mesh->getNodesNum(), nt = mesh->getTetsNum();
+  // clamp the nodes at the smallest x (the reference's own rule picks none on this mesh)
+  float xmin = 1e30f;
+  for (int i = 0; i < nn; ++i) xmin = std::min(xmin, mesh->getRestPosition().v[3 * i]);
+  mesh->getConstraintIDs().clear();
+  for (int i = 0; i < nn; ++i)
+    if (mesh->getRestPosition().v[3 * i] == xmin) mesh->getConstraintIDs().push_back(i);
+  Mat<float> pos = mesh->getRestPosition();
+  for (size_t k = 0; k < pos.v.size(); ++k) pos.v[k] += 0.001f * std::sin(0.7f * (float)k);  // no tet inverts
+  mesh->getNodes() = pos;
+  mesh->getExternalForces().v[3 * (nn - 1) + 1] = 0.25f;
+  phybs::IsotropicNeohookeanMaterialT<DemoTetMesh> mat(mesh);
+  Mat<float> f = mat.computeInnerForceFromPos(pos);
+  mat.computeElasticEnergyFromPos(pos);
+  phybs::CsrMatrix K = mat.computeGlobalStiffnessMatrixFromPos(pos);
+  Mat<float> k0 = mat.computeStiffnessMatrixFromPos(0, pos), k1 = mat.computeStiffnessMatrixFromPos(nt - 1, pos);
+  const float fh[3] = {1.1f, 0.95f, 0.9f};
+  float grad[3], hess[9];
+  mat.computeEnergy2FhatGradient(0, fh, grad);
+  mat.computeEnergy2FhatHessian(0, fh, hess);
+  FILE *o = std::fopen(out, "wb");
+  if (!o) return 1;
+  put(o, mesh->getRestPosition().data(), 3 * (size_t)nn);
+  put(o, mesh->getTets().data(), 4 * (size_t)nt);
+  put(o, mesh->getFaces().data(), 3 * (size_t)mesh->getFacesNum());
+  put(o, mesh->getMasses().data(), nn);
+  put(o, mesh->getInvMasses().data(), nn);
+  put(o, mesh->getTetVolumes().data(), nt);
+  put(o, mesh->getDmInvs().data(), 9 * (size_t)nt);
+  put(o, mesh->getGravities().data(), 3 * (size_t)nn);
+  put(o, pos.data(), 3 * (size_t)nn);
+  put(o, f.data(), 3 * (size_t)nn);
+  put(o, mat.getElasticEnergys().data(), nt);
+  put(o, K.rowptr.data(), K.rowptr.size());
+  put(o, K.col.data(), K.col.size());
+  put(o, K.val.data(), K.val.size());
+  put(o, mat.getDiagonalIdx().data(), mat.getDiagonalIdx().size());
+  put(o, mat.getKIDinCSRval().data(), mat.getKIDinCSRval().size());
+  put(o, k0.data(), 144);
+  put(o, k1.data(), 144);
+  put(o, grad, 3);
+  put(o, hess, 9);
+  put(o, mesh->getConstraintIDs().data(), mesh->getConstraintIDs().size());
+  std::fclose(o);
+  return 0;
+}
 
 // A mesh type with the TetMesh getters that DescentOptimizeT uses (TetMesh.h:41-56).
 struct DemoMesh {
@@ -58,8 +133,17 @@ struct DemoMaterial {};
 
 int main(int argc, char **argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: compat_demo cuda|descent|backeuler|timeint <steps>\n");
+    std::fprintf(stderr, "usage: compat_demo cuda|descent|descent_stretch|backeuler|timeint <steps> | material <prefix> <out>\n");
     return 2;
+  }
+  if (std::strcmp(argv[1], "material") == 0) {
+    if (argc < 4) return 2;
+    try {
+      return material_mode(argv[2], argv[3]);
+    } catch (const phybs::Error &e) {
+      std::fprintf(stderr, "%s\n", e.what());
+      return 1;
+    }
   }
   if (std::strcmp(argv[1], "timeint") == 0) {
     const int nx = 4, ny = 4, nz = 16, nn = (nx + 1) * (ny + 1) * (nz + 1), nt = 6 * nx * ny * nz;
@@ -96,6 +180,7 @@ int main(int argc, char **argv) {
     return 0;
   }
   const bool cuda = std::strcmp(argv[1], "cuda") == 0;
+  const bool stretch = std::strcmp(argv[1], "descent_stretch") == 0;
   const bool backeuler = std::strcmp(argv[1], "backeuler") == 0;
   const int steps = std::atoi(argv[2]);
   const int nx = 4, ny = 4, nz = 16;
@@ -136,8 +221,20 @@ int main(int argc, char **argv) {
       mesh->tets.v = tets;
       for (int i = 0; i < nn; ++i)
         if (mask[i]) mesh->cons.push_back(i);
-      phybs::DescentOptimizeT<DemoMesh, DemoMaterial> opt(mesh, std::make_shared<DemoMaterial>());
-      for (int s = 0; s < steps; ++s) opt.doDescentOpt(96);
+      if (stretch) {
+        // the material object decides the energy: the stretch material's youngs() / poisson() /
+        // materialKind() reach the solver
+        struct StretchMaterial {
+          float youngs() const { return 2e5f; }
+          float poisson() const { return 0.4f; }
+          int materialKind() const { return PBS_MAT_STRETCH; }
+        };
+        phybs::DescentOptimizeT<DemoMesh, StretchMaterial> opt(mesh, std::make_shared<StretchMaterial>());
+        for (int s = 0; s < steps; ++s) opt.doDescentOpt(96);
+      } else {
+        phybs::DescentOptimizeT<DemoMesh, DemoMaterial> opt(mesh, std::make_shared<DemoMaterial>());
+        for (int s = 0; s < steps; ++s) opt.doDescentOpt(96);
+      }
       out = mesh->nodes.v;
     }
   } catch (const phybs::Error &e) {

@@ -1,6 +1,7 @@
 """GPU tests of the learned warm start: the MFMA network and the training kernels against the numpy
 specification (oracle/mlp_oracle.py), the features, the fallback, the warm-started step against
 the CPU oracle's warm-started step, and what a trained network gains."""
+import os
 import sys
 
 import numpy as np
@@ -225,63 +226,118 @@ def test_collect_and_train_on_gpu():
     s.close()
 
 
+def bench_params(**kw):
+    return P.default_params(precision=P.FP64, relgrad_tol=1e-4, max_iterations=20000, dt=1 / 30.0,
+                            grav=(0.0, 0.0, -9.8), profile=[(1, 0.96), (7, 0.991), (10, 0.9995)], **kw)
+
+
+def warm_run(rest, tets, mask, prm, x0, steps, mdl=None, collect=0, stride=1, seed=None, sweeps=10, warmup=0):
+    """steps steps from x0 (impulses on 10 random nodes every 10th step when seed is given);
+    returns (solver, stats of the steps after the warm-up)."""
+    s = P.Solver(rest, tets, mask, prm)
+    s.set_positions(x0)
+    if collect:
+        s.learn_collect(collect, stride)
+    if mdl is not None:
+        s.learn_set_model(*mdl)
+        s.learn_enable(True)
+        s.learn_set_options(sweeps, (1.0,))
+    r = np.random.default_rng(seed) if seed is not None else None
+    st = []
+    for k in range(warmup + steps):
+        ext = None
+        if r is not None and k % 10 == 5:
+            ext = np.zeros_like(rest)
+            ext[r.integers(0, len(rest), 10)] = r.normal(0, 20, (10, 3))
+        q = s.step(ext)
+        if k >= warmup:
+            st.append(q)
+    return s, st
+
+
 def test_trained_warm_start_gain():
-    """What the trained network gains: train on a seeded trajectory (twist, impulses), then run a
-    different seed with and without the warm start at rel-grad 1e-4 and report the iterations.
-    The gradient test guarantees that a worse guess is never used."""
-    rest, tets = M.bar(8, 8, 32, 0.05)
-    mask = M.clamp_x0(rest)
-    prm = P.default_params(precision=P.FP64, relgrad_tol=1e-4, max_iterations=20000, dt=1 / 30.0,
-                           grav=(0.0, 0.0, -9.8), profile=[(1, 0.96), (7, 0.991), (10, 0.9995)])
+    """What the trained network gains on the benchmark's own configuration: the 64x64x256 bar
+    (6.3 M tets) sagging from rest under gravity, rel-grad 1e-4.  The network is trained on the
+    plain solver's own trajectory from a held-out start (0.2 % jitter, seed 7, impulses; every
+    4th free node of 30 steps, 8 M pairs), then the evaluation trajectory (from rest, no jitter)
+    runs with and without the warm start (5 warm-up steps, 10 measured): the guess is used in at
+    least 80 % of the steps and saves at least a quarter of the iterations, and both runs reach
+    rel-grad 1e-4 in every step.  The gradient test guarantees that a worse guess is never used."""
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
 
-    def run(seed, steps, mdl=None, collect=0):
-        s = P.Solver(rest, tets, mask, prm)
-        s.set_positions(M.twisted_state(rest, 8, 8, 32, 0.05, 60.0, 0.01, seed))
-        if collect:
-            s.learn_collect(collect)
-        if mdl is not None:
-            s.learn_set_model(*mdl)
-            s.learn_enable(True)
-            s.learn_set_options(100, (1.0, 0.5, 0.25, 0.1))
-        r = np.random.default_rng(seed)
-        st = []
-        for k in range(steps):
-            ext = None
-            if k % 10 == 5:
-                ext = np.zeros_like(rest)
-                ext[r.integers(0, len(rest), 10)] = r.normal(0, 20, (10, 3))
-            st.append(s.step(ext))
-        return s, st
-
-    s, _ = run(1, 60, collect=60 * len(rest))
+    dims = (64, 64, 256)
+    rest, tets = P.mesh_bar(*dims, 0.05)
+    mask = (rest[:, 0] == 0).astype(np.int32)
+    prm = bench_params()
+    say("64x64x256 warm start: collecting the training trajectory")
+    s = bench.collect_trajectory(P, rest, tets, mask, P.mesh_twist(rest, *dims, 0.05, 0.0, 0.002, 7), prm, 30, 7, 4,
+                                 12_000_000)
     s.learn_fit_scales()
     s.learn_init_model(7)
-    loss = s.learn_train(20, batch=2048, lr=2e-3, seed=1)
+    loss = s.learn_train(30, batch=8192, lr=2e-3, seed=7)
     mdl = s.learn_get_model()
     s.close()
-    sp, stp = run(11, 30)
-    sl, stl = run(11, 30, mdl)
-    # the same network under the mixed mode (FP32 forces down to |g|/|g0| = 2e-4): every step
-    # still converges on FP64 passes, with the warm-start decision of the FP64 run's first step
-    prm_fp64 = prm
-    prm = P.default_params(precision=P.FP64, relgrad_tol=1e-4, max_iterations=20000, dt=1 / 30.0,
-                           grav=(0.0, 0.0, -9.8), profile=[(1, 0.96), (7, 0.991), (10, 0.9995)], mixed_switch=2e-4)
-    sm, stm = run(11, 10, mdl)
-    prm = prm_fp64
-    assert all(q.converged and q.g_norm <= 1e-4 * q.g0_norm for q in stm)
-    assert stm[0].used_learned == stl[0].used_learned
-    say("mixed + warm start: iterations/step %.1f (FP64: %.1f over the same 10 steps)"
-        % (np.mean([q.iterations for q in stm]), np.mean([q.iterations for q in stl[:10]])))
-    sm.close()
+    sp, stp = warm_run(rest, tets, mask, prm, rest, 10, warmup=5)
+    sl, stl = warm_run(rest, tets, mask, prm, rest, 10, mdl, warmup=5, sweeps=20)
     ip = np.mean([q.iterations for q in stp])
     il = np.mean([q.iterations for q in stl])
     used = np.mean([q.used_learned for q in stl])
-    say("trained: loss %.4f -> %.4f; iterations/step plain %.1f, warm-started %.1f (%.1f%%), guess used in %.0f%% of steps"
-        % (loss[0], loss[-1], ip, il, 100 * (ip - il) / ip, 100 * used))
+    say("trained: loss %.4f -> %.4f; iterations/step plain %.1f, warm-started %.1f (%.1f%% saved), guess used in "
+        "%.0f%% of steps" % (loss[0], loss[-1], ip, il, 100 * (ip - il) / ip, 100 * used))
     assert loss[-1] < 0.2 * loss[0]
-    assert all(q.converged for q in stl)
-    # the warm start never costs more than a few percent (different trajectories round
-    # differently); what it gains is the reported number
-    assert il <= 1.03 * ip
+    assert all(q.converged and q.g_norm <= 1e-4 * q.g0_norm for q in stl + stp)
+    assert used >= 0.8
+    assert il <= 0.75 * ip
+    xp, xl = sp.positions(f64=True), sl.positions(f64=True)
+    say("|x_learned - x_plain| after 15 steps: %.2e m (bar length 12.8 m)" % np.abs(xl - xp).max())
+    assert np.abs(xl - xp).max() <= 1e-4 * 12.8
+    sp.close()
+    sl.close()
+    # the same network under the mixed mode (FP32 forces down to |g|/|g0| = 2e-4): every step
+    # still converges on FP64 passes, with the warm-start decision of the FP64 run's first step
+    sm, stm = warm_run(rest, tets, mask, bench_params(mixed_switch=2e-4), rest, 5, mdl, warmup=5, sweeps=20)
+    assert all(q.converged and q.g_norm <= 1e-4 * q.g0_norm for q in stm)
+    assert stm[0].used_learned == stl[0].used_learned
+    say("mixed + warm start: iterations/step %.1f (FP64: %.1f over the same 5 steps)"
+        % (np.mean([q.iterations for q in stm]), np.mean([q.iterations for q in stl[:5]])))
+    sm.close()
+
+
+def test_config1_warm_start_protocol():
+    """Configuration 4's protocol evaluated on configuration 1: the 12x12x50 bar released from a
+    90 degree twist (1 % jitter, seed 11), network trained on the plain solver's trajectories from
+    other seeds (2 and 3, impulses).  On this configuration the network does not beat the plain
+    guess: its predicted corrections carry short-wavelength errors that the gradient test sees,
+    and the guess is rejected in most steps.  What the test holds: every step converges to
+    rel-grad 1e-4, the warm start costs at most 2 % of the plain iterations, and the trajectory
+    matches the plain one to the solver's tolerance.  The iterations saved are printed."""
+    dims = (12, 12, 50)
+    rest, tets = M.bar(*dims, 0.05)
+    mask = M.clamp_x0(rest)
+    prm = bench_params()
+    s, _ = warm_run(rest, tets, mask, prm, M.twisted_state(rest, *dims, 0.05, 90.0, 0.01, 2), 60,
+                    collect=2_000_000, seed=2)
+    q, _ = warm_run(rest, tets, mask, prm, M.twisted_state(rest, *dims, 0.05, 90.0, 0.01, 3), 60,
+                    collect=1_000_000, seed=3)
+    f, t = q.learn_get_samples()
+    q.close()
+    s.learn_add_samples(f, t)
+    s.learn_fit_scales()
+    s.learn_init_model(7)
+    s.learn_train(30, batch=8192, lr=2e-3, seed=7)
+    mdl = s.learn_get_model()
+    s.close()
+    x0 = M.twisted_state(rest, *dims, 0.05, 90.0, 0.01, 11)
+    sp, stp = warm_run(rest, tets, mask, prm, x0, 30, warmup=1)
+    sl, stl = warm_run(rest, tets, mask, prm, x0, 30, mdl, warmup=1)
+    ip = np.mean([q.iterations for q in stp])
+    il = np.mean([q.iterations for q in stl])
+    used = np.mean([q.used_learned for q in stl])
+    say("config 1: iterations/step plain %.1f, warm-started %.1f (%.2f%% saved), guess used in %.0f%% of steps"
+        % (ip, il, 100 * (ip - il) / ip, 100 * used))
+    assert all(q.converged and q.g_norm <= 1e-4 * q.g0_norm for q in stl)
+    assert il <= 1.02 * ip
+    assert np.abs(sl.positions(f64=True) - sp.positions(f64=True)).max() <= 1e-4 * 2.5
     sp.close()
     sl.close()
