@@ -222,6 +222,12 @@ int pbs_stiffness_matrix(pbs_solver *s, int64_t nnz, double *csr_val);
  * 3b + e for the tet's corners a, b in its own corner order and axes d, e) = -df/dx of the tet's
  * elastic forces at positions pos (3N AoS).  Single-mesh handles only. */
 int pbs_tet_stiffness(pbs_solver *s, const float *pos, int num, const int *tet_ids, float *K);
+/* No counterpart in the reference.  pbs_tet_stiffness without any device allocation, for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops): its device buffers
+ * (tet ids, matrices) are a fixed 4.75 MB array per GPU, held until the process ends.  At most
+ * 8192 tets per call (PBS_ERR_ARG beyond).  Calls that overlap on one GPU share that array and
+ * race; results equal pbs_tet_stiffness's. */
+int pbs_tet_stiffness_noalloc(pbs_solver *s, const float *pos, int num, const int *tet_ids, float *K);
 
 /* ---- projective dynamics (ProjDynamic::doProjDynamics, ProjDynamic.cpp:442-479) ---- */
 typedef struct pbs_pd_params {

@@ -132,6 +132,7 @@ SIGNATURES = {
     "pbs_tetgen_read_tets": (C.c_int, [C.c_char_p, _P, _P]),
     "pbs_tetgen_read_faces": (C.c_int, [C.c_char_p, _P, _P]),
     "pbs_tet_stiffness": (C.c_int, [_P, _P, C.c_int, _P, _P]),
+    "pbs_tet_stiffness_noalloc": (C.c_int, [_P, _P, C.c_int, _P, _P]),
     "pbs_node_write": (C.c_int, [C.c_char_p, C.c_int, _P]),
     "pbs_node_read": (C.c_int, [C.c_char_p, C.c_int, _P]),
     "pbs_csr_pattern": (C.c_int64, [C.c_int, C.c_int, _P, _P, _P, _P, _P]),
@@ -440,12 +441,14 @@ class Solver:
         _check(lib().pbs_stiffness_matrix(self.h, len(col), _p(val)))
         return rowptr, col, val
 
-    def tet_stiffness(self, pos, tet_ids):
-        """Per-tet 12 x 12 stiffness K = -df/dx of the listed tets at pos (FP32)."""
+    def tet_stiffness(self, pos, tet_ids, noalloc=False):
+        """Per-tet 12 x 12 stiffness K = -df/dx of the listed tets at pos (FP32); noalloc: through
+        pbs_tet_stiffness_noalloc (no device allocation, at most 8192 tets)."""
         pos = _f32(pos)
         ids = np.ascontiguousarray(tet_ids, np.int32).reshape(-1)
         K = np.zeros((len(ids), 12, 12), np.float32)
-        _check(lib().pbs_tet_stiffness(self.h, _p(pos), len(ids), _p(ids), _p(K)))
+        fn = lib().pbs_tet_stiffness_noalloc if noalloc else lib().pbs_tet_stiffness
+        _check(fn(self.h, _p(pos), len(ids), _p(ids), _p(K)))
         return K
 
     def do_descent_opt(self, h, iterations, ext):

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -458,13 +459,38 @@ template <typename R> int stiffness_impl(pbs_solver *s, int64_t nnz, double *csr
   return PBS_OK;
 }
 
-template <typename R> int tet_stiffness_impl(pbs_solver *s, const float *pos, int num, const int *ids, float *K) {
+// Per-tet stiffness of the listed tets (caller ids, already checked and turned into solver ids
+// in sid) through the device buffers dids (num ints) and dK (144 num floats).
+template <typename R>
+int tet_stiffness_run(pbs_solver *s, const float *pos, int num, const int *sid, int *dids, float *dK, float *K) {
   DevMesh<R> &m = mesh<R>(s);
-  std::vector<int> sid(std::max(num, 1));
+  hipError_t e = hipMemcpyAsync(s->stage, pos, sizeof(float) * 3 * (size_t)s->nn, hipMemcpyHostToDevice, s->st);
+  if (e == hipSuccess) {
+    Launch<R>::aos_to_planes(s->stage, m.ring.b[0], m.perm, s->nn, s->st);
+    e = hipMemcpyAsync(dids, sid, sizeof(int) * num, hipMemcpyHostToDevice, s->st);
+  }
+  if (e == hipSuccess) {
+    IeLaunch<R>::tet_stiffness(m, s->dp, s->td.inst_mu, s->td.inst_lam, m.ring.b[0], num, dids, dK, s->st);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(K, dK, sizeof(float) * 144 * (size_t)num, hipMemcpyDeviceToHost, s->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+  if (e != hipSuccess) return set_error(PBS_ERR_HIP, std::string("pbs_tet_stiffness: ") + hipGetErrorString(e));
+  return PBS_OK;
+}
+
+int tet_ids_to_solver(pbs_solver *s, int num, const int *ids, int *sid) {
   for (int k = 0; k < num; ++k) {
     if (ids[k] < 0 || ids[k] >= s->nt) return set_error(PBS_ERR_ARG, "pbs_tet_stiffness: tet id out of range");
     sid[k] = s->plan.itperm[ids[k]];
   }
+  return PBS_OK;
+}
+
+template <typename R> int tet_stiffness_impl(pbs_solver *s, const float *pos, int num, const int *ids, float *K) {
+  std::vector<int> sid(std::max(num, 1));
+  int rc = tet_ids_to_solver(s, num, ids, sid.data());
+  if (rc) return rc;
   int *dids = nullptr;
   float *dK = nullptr;
   HIPCHK(hipMalloc(&dids, sizeof(int) * sid.size()));
@@ -473,21 +499,52 @@ template <typename R> int tet_stiffness_impl(pbs_solver *s, const float *pos, in
     (void)hipFree(dids);
     return set_error(PBS_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
   }
-  e = hipMemcpyAsync(s->stage, pos, sizeof(float) * 3 * (size_t)s->nn, hipMemcpyHostToDevice, s->st);
-  if (e == hipSuccess) {
-    Launch<R>::aos_to_planes(s->stage, m.ring.b[0], m.perm, s->nn, s->st);
-    e = hipMemcpyAsync(dids, sid.data(), sizeof(int) * num, hipMemcpyHostToDevice, s->st);
-  }
-  if (e == hipSuccess) {
-    IeLaunch<R>::tet_stiffness(m, s->dp, s->td.inst_mu, s->td.inst_lam, m.ring.b[0], num, dids, dK, s->st);
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) e = hipMemcpyAsync(K, dK, sizeof(float) * 144 * (size_t)num, hipMemcpyDeviceToHost, s->st);
-  if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+  rc = tet_stiffness_run<R>(s, pos, num, sid.data(), dids, dK, K);
   (void)hipFree(dids);
   (void)hipFree(dK);
-  if (e != hipSuccess) return set_error(PBS_ERR_HIP, std::string("pbs_tet_stiffness: ") + hipGetErrorString(e));
+  return rc;
+}
+
+// ---- the variant of pbs_tet_stiffness that allocates nothing ----
+// Its device buffers (the listed tets' solver ids, then their 12 x 12 matrices) live in one
+// file-scope device array, one copy per GPU, held until the process ends.  Host code never uses
+// the array's name as an address: the address is looked up per device with
+// hipGetSymbolAddress on that device's first call and kept in scratch_ptr.
+constexpr int kScratchTets = 8192;  // tets per call
+constexpr size_t kScratchIdBytes = (sizeof(int) * kScratchTets + 255) / 256 * 256;
+constexpr size_t kScratchBytes = kScratchIdBytes + sizeof(float) * 144 * (size_t)kScratchTets;  // 4.75 MB
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void *scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(int device, unsigned char **out) {
+  if (device < 0 || device >= kMaxDevices) return set_error(PBS_ERR_STATE, "device index out of range");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[device]) {
+    int cur = -1;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != device) HIPCHK(hipSetDevice(device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      return set_error(PBS_ERR_HIP, std::string("scratch address (hipGetSymbolAddress): ") + hipGetErrorString(e));
+    scratch_ptr[device] = p;
+  }
+  *out = (unsigned char *)scratch_ptr[device];
   return PBS_OK;
+}
+
+template <typename R> int tet_stiffness_noalloc_impl(pbs_solver *s, const float *pos, int num, const int *ids, float *K) {
+  int sid[kScratchTets];
+  int rc = tet_ids_to_solver(s, num, ids, sid);
+  if (rc) return rc;
+  unsigned char *base = nullptr;
+  rc = scratch_address(s->device, &base);
+  if (rc) return rc;
+  return tet_stiffness_run<R>(s, pos, num, sid, (int *)base, (float *)(base + kScratchIdBytes), K);
 }
 
 }  // namespace
@@ -518,6 +575,18 @@ extern "C" int pbs_tet_stiffness(pbs_solver *s, const float *pos, int num, const
   if (num == 0) return PBS_OK;
   return s->p.precision == PBS_FP64 ? tet_stiffness_impl<double>(s, pos, num, tet_ids, K)
                                     : tet_stiffness_impl<float>(s, pos, num, tet_ids, K);
+}
+
+extern "C" int pbs_tet_stiffness_noalloc(pbs_solver *s, const float *pos, int num, const int *tet_ids, float *K) {
+  if (!s || !pos || num < 0 || (num > 0 && (!tet_ids || !K)))
+    return set_error(PBS_ERR_ARG, "pbs_tet_stiffness_noalloc: bad arguments");
+  if (num > kScratchTets)
+    return set_error(PBS_ERR_ARG, "pbs_tet_stiffness_noalloc: at most " + std::to_string(kScratchTets) +
+                                      " tets per call (" + std::to_string(num) + " given)");
+  if (s->ninst != 1) return set_error(PBS_ERR_STATE, "pbs_tet_stiffness_noalloc: single-mesh handles only");
+  if (num == 0) return PBS_OK;
+  return s->p.precision == PBS_FP64 ? tet_stiffness_noalloc_impl<double>(s, pos, num, tet_ids, K)
+                                    : tet_stiffness_noalloc_impl<float>(s, pos, num, tet_ids, K);
 }
 
 extern "C" int pbs_stiffness_matrix(pbs_solver *s, int64_t nnz, double *csr_val) {
