@@ -12,6 +12,7 @@
 // vectors are 3 planes (x | y | z) of nn values (SoA); per-tet rest data are 10 planes (Dm^-1
 // entries, volume) plus 16 bytes per tet of tile-local ids and contribution slots for the tile
 // pass; per-tet force records of the standalone calls are 12 consecutive values.
+#include <algorithm>
 #include <type_traits>
 
 #include "pbs_device.h"
@@ -87,23 +88,18 @@ __device__ __forceinline__ void cheb_schedule(const DevParams &p, int k, float &
 // formulas written out without fused multiply-adds.
 // ---------------------------------------------------------------------------------------------
 #pragma clang fp contract(off)
-__global__ void k_setup_tets(DevParams p, const float *__restrict__ rest, const int4 *__restrict__ tets,
-                             float *__restrict__ dmi, float *__restrict__ vol, int *__restrict__ flag) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= p.nt) return;
-  const int N = p.nn, T = p.nt;
-  const int4 q = tets[t];
-  if (q.x < 0 || q.y < 0 || q.z < 0 || q.w < 0 || q.x >= N || q.y >= N || q.z >= N || q.w >= N) {
-    atomicOr(flag, 1);
-    return;
-  }
+// Dm^-1 (row-major) and the volume |det Dm| / 6 of a rest tet from its corners' rest positions
+// r[c][k] (compute_DmInv_ANs_Mass, CUDAInterface.cu:273-325; TetMesh.cpp:189-216); returns det Dm.
+// The setup kernel and the tile pass both call this, so the tile pass's values are bitwise the
+// stored ones.
+__device__ __forceinline__ float rest_shape(const float r[4][3], float Dmi[9], float &V) {
   float Dm[9], C[9];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const float x0 = rest[k * N + q.x];
-    Dm[3 * k + 0] = rest[k * N + q.y] - x0;
-    Dm[3 * k + 1] = rest[k * N + q.z] - x0;
-    Dm[3 * k + 2] = rest[k * N + q.w] - x0;
+    const float x0 = r[0][k];
+    Dm[3 * k + 0] = r[1][k] - x0;
+    Dm[3 * k + 1] = r[2][k] - x0;
+    Dm[3 * k + 2] = r[3][k] - x0;
   }
   // adjugate and determinant written out here so that the no-contraction pragma applies
   C[0] = Dm[4] * Dm[8] - Dm[7] * Dm[5];
@@ -116,11 +112,35 @@ __global__ void k_setup_tets(DevParams p, const float *__restrict__ rest, const 
   C[7] = Dm[1] * Dm[6] - Dm[0] * Dm[7];
   C[8] = Dm[0] * Dm[4] - Dm[1] * Dm[3];
   const float det = Dm[0] * C[0] + Dm[3] * C[1] + Dm[6] * C[2];
-  if (!(det > 0.0f)) atomicOr(flag, 2);
-  vol[t] = fabsf(det) / 6.0f;
+  V = fabsf(det) / 6.0f;
   const float inv = 1.0f / det;
 #pragma unroll
-  for (int i = 0; i < 9; ++i) dmi[(size_t)i * T + t] = C[i] * inv;
+  for (int i = 0; i < 9; ++i) Dmi[i] = C[i] * inv;
+  return det;
+}
+
+__global__ void k_setup_tets(DevParams p, const float *__restrict__ rest, const int4 *__restrict__ tets,
+                             float *__restrict__ dmi, float *__restrict__ vol, int *__restrict__ flag) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= p.nt) return;
+  const int N = p.nn, T = p.nt;
+  const int4 q = tets[t];
+  if (q.x < 0 || q.y < 0 || q.z < 0 || q.w < 0 || q.x >= N || q.y >= N || q.z >= N || q.w >= N) {
+    atomicOr(flag, 1);
+    return;
+  }
+  const int qn[4] = {q.x, q.y, q.z, q.w};
+  float r[4][3];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r[c][k] = rest[k * N + qn[c]];
+  float Dmi[9], V;
+  const float det = rest_shape(r, Dmi, V);
+  if (!(det > 0.0f)) atomicOr(flag, 2);
+  vol[t] = V;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) dmi[(size_t)i * T + t] = Dmi[i];
 }
 
 // Lumped mass gathered over the incidence list in ascending tet order (the reference's float
@@ -338,9 +358,40 @@ __device__ __forceinline__ void tile_tet(int phase, const uint4 lp, const float 
       else if (clamp == 2) er = er > (R)1e-6 ? er : (R)0;  // IsotropicNeohookeanMaterial.cpp:129
       e_el += (double)er;
     }
+  } else if (MAT == 0) {
+    // the neo-Hookean stiffness diagonal of neo_tet<C, true>, written corner by corner as it is
+    // formed (the 12 values are never all live: this phase sets the kernel's register count)
+    C Cf[9];
+    const C J = adj3(F, Cf);
+    const C iJ = (C)1 / J;
+    C Fi[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fi[i] = Cf[i] * iJ;
+    const C k2 = mu + lam - lam * dlog(J);
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      C c3[3];
+      if (n == 0) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) c3[j] = -((C)Dmf[j] + (C)Dmf[3 + j] + (C)Dmf[6 + j]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) c3[j] = (C)Dmf[3 * (n - 1) + j];
+      }
+      const C cc = c3[0] * c3[0] + c3[1] * c3[1] + c3[2] * c3[2];
+      C h[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const C sd = c3[0] * Fi[d] + c3[1] * Fi[3 + d] + c3[2] * Fi[6 + d];
+        h[d] = V * (mu * cc + k2 * sd * sd);
+      }
+      C *r = rec + W * q[n];
+      r[0] = (C)(float)h[0];
+      r[1] = (C)(float)h[1];
+      r[2] = (C)(float)h[2];
+    }
   } else {
-    if (MAT == 0) neo_tet<C, true>(F, Dmi, V, mu, lam, f12, en, h12);
-    else stretch_tet<C, true>(F, Dmi, V, mu, lam, f12, en, h12);
+    stretch_tet<C, true>(F, Dmi, V, mu, lam, f12, en, h12);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       C *r = rec + W * q[c];
@@ -399,6 +450,13 @@ __device__ __forceinline__ void tile_gather(int phase, const TileInfo &ti, const
   }
 }
 
+// Byte offset of the tile pass's rest-position region in LDS (after the positions and the
+// contribution slots; 16-byte aligned).
+template <typename R> __host__ __device__ inline size_t tile_rest_offset(const TileDev &td) {
+  const size_t b = (4 * (size_t)td.lcap + 4 * (size_t)ContribWidth<R>::value * td.tcap) * sizeof(R);
+  return (b + 15) & ~(size_t)15;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Tile pass (per-tet half of computeForceKd, CUDAInterface.cu:495-568): one workgroup per tile
 // evaluates each of the tile's tets once -- force, energy on check passes, Hessian diagonal on
@@ -406,33 +464,33 @@ __device__ __forceinline__ void tile_gather(int phase, const TileInfo &ti, const
 // array in which each local node's contributions form one contiguous run (ordered by tet, then
 // corner).  The gather then sums each run in that fixed order (no atomics) and writes one partial
 // per owned node (fown) and per halo node (hpart).
-//   LDS: positions of local nodes (4 R each), contributions (4 per tet, W values each).
-//   HBM per tet: 16 B of local ids / slots, 40 B of rest data from the SoA planes.
+//   LDS: positions of local nodes (4 R each), contributions (4 per tet, W values each), rest
+//   positions of local nodes (float4 each).
+//   HBM per tet: 16 B of local ids / slots.  Dm^-1 and the volume are not streamed from memory
+//   (40 B per tet): they are recomputed from the staged rest positions with the setup kernel's
+//   own FP32 arithmetic (rest_shape), so they are bitwise the stored values.
 // ---------------------------------------------------------------------------------------------
-// F32M (FP64 state only): the per-tet arithmetic in FP32 after F (the mixed mode's early passes).
+// One tile, all phases: stage the local nodes' positions and rest positions, evaluate the tets
+// (phase 0 forces and energies; phase 1, when hd, the stiffness diagonal), gather each phase.
 template <typename R, int MAT, bool F32M>
-__global__ void __launch_bounds__(256)
-k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, const float *__restrict__ dmi,
-       const float *__restrict__ vol, R *__restrict__ fown, R *__restrict__ hpart, float *__restrict__ fownh,
-       float *__restrict__ hparth) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double lds_red[16];
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const TileInfo ti = td.tiles[b];
-  ctrl += ti.inst;
-  if (ctrl->done) return;
-  const int hdnow = ctrl->hd_now, chk = ctrl->check_now;
+__device__ __forceinline__ void tile_body(const DevParams &p, const Ctrl *ctrl, const Ring<R> &ring, const TileDev &td,
+                                          int b, const TileInfo &ti, int hdnow, const float *__restrict__ rest,
+                                          R *__restrict__ fown, R *__restrict__ hpart, float *__restrict__ fownh,
+                                          float *__restrict__ hparth, unsigned char *smem, double *lds_red) {
+  const int chk = ctrl->check_now;
   const R *__restrict__ x = ring.b[ctrl->cur];
-  const int L = ti.own_cnt + ti.halo_cnt, N = p.nn, T = p.nt, tid = threadIdx.x;
+  const int L = ti.own_cnt + ti.halo_cnt, N = p.nn, tid = threadIdx.x;
   typedef typename std::conditional<F32M, float, R>::type C;
   R *sx = (R *)smem;
   C *rec = (C *)(sx + 4 * td.lcap);
+  float4 *srest = (float4 *)(smem + tile_rest_offset<R>(td));
   const int nthr = blockDim.x;
   for (int i = tid; i < L; i += nthr) {
     const int g = i < ti.own_cnt ? ti.own_start + i : td.halo[ti.halo_off + i - ti.own_cnt];
     sx[4 * i] = x[g];
     sx[4 * i + 1] = x[N + g];
     sx[4 * i + 2] = x[2 * (size_t)N + g];
+    srest[i] = make_float4(rest[g], rest[N + g], rest[2 * (size_t)N + g], 0.0f);
   }
   __syncthreads();
   const C mu = (C)td.inst_mu[ti.inst], lam = (C)td.inst_lam[ti.inst];
@@ -441,10 +499,18 @@ k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
     for (int s = tid; s < ti.tet_cnt; s += nthr) {
       const size_t t = (size_t)ti.tet_start + s;
       const uint4 lp = td.tet_lp[t];
-      float Dmf[9];
+      const int lid[4] = {(int)(lp.x & 0xffff), (int)(lp.x >> 16), (int)(lp.y & 0xffff), (int)(lp.y >> 16)};
+      float r[4][3];
 #pragma unroll
-      for (int k = 0; k < 9; ++k) Dmf[k] = dmi[k * (size_t)T + t];
-      tile_tet<R, C, MAT>(phase, lp, Dmf, vol[t], sx, rec, mu, lam, chk, p.clamp, e_el);
+      for (int c = 0; c < 4; ++c) {
+        const float4 q = srest[lid[c]];
+        r[c][0] = q.x;
+        r[c][1] = q.y;
+        r[c][2] = q.z;
+      }
+      float Dmf[9], Vf;
+      rest_shape(r, Dmf, Vf);
+      tile_tet<R, C, MAT>(phase, lp, Dmf, Vf, sx, rec, mu, lam, chk, p.clamp, e_el);
     }
     __syncthreads();
     tile_gather<R>(phase, ti, td, rec, N, fown, hpart, fownh, hparth);
@@ -454,6 +520,20 @@ k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
     const double se = block_sum(e_el, lds_red);
     if (tid == 0) td.tpart[b] = se;
   }
+}
+
+// F32M (FP64 state only): the per-tet arithmetic in FP32 after F (the mixed mode's early passes).
+template <typename R, int MAT, bool F32M>
+__global__ void __launch_bounds__(256)
+k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, const float *__restrict__ rest,
+       R *__restrict__ fown, R *__restrict__ hpart, float *__restrict__ fownh, float *__restrict__ hparth) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double lds_red[16];
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const TileInfo ti = td.tiles[b];
+  ctrl += ti.inst;
+  if (ctrl->done) return;
+  tile_body<R, MAT, F32M>(p, ctrl, ring, td, b, ti, ctrl->hd_now, rest, fown, hpart, fownh, hparth, smem, lds_red);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -797,24 +877,20 @@ void Launch<R>::tet(const DevMesh<R> &m, const TileDev &td, const DevParams &p, 
 }
 template <typename R>
 size_t Launch<R>::tile_lds(const TileDev &td) {
-  return (4 * (size_t)td.lcap + 4 * (size_t)ContribWidth<R>::value * td.tcap) * sizeof(R);
+  return tile_rest_offset<R>(td) + 16 * (size_t)td.lcap;
 }
 template <typename R>
 void Launch<R>::tile_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
   const size_t lds = tile_lds(td);
   const bool f32m = sizeof(R) == 8 && p.f32math;
-  if (p.material == 0 && !f32m)
-    hipLaunchKernelGGL((k_tile<R, 0, false>), dim3(td.ntiles), dim3(td.threads), lds, st, p, m.ctrl, m.ring, td, m.dmi,
-                       m.vol, m.fown, m.hpart, m.fownh, m.hparth);
-  else if (p.material == 0)
-    hipLaunchKernelGGL((k_tile<R, 0, true>), dim3(td.ntiles), dim3(td.threads), lds, st, p, m.ctrl, m.ring, td, m.dmi,
-                       m.vol, m.fown, m.hpart, m.fownh, m.hparth);
-  else if (!f32m)
-    hipLaunchKernelGGL((k_tile<R, 1, false>), dim3(td.ntiles), dim3(td.threads), lds, st, p, m.ctrl, m.ring, td, m.dmi,
-                       m.vol, m.fown, m.hpart, m.fownh, m.hparth);
-  else
-    hipLaunchKernelGGL((k_tile<R, 1, true>), dim3(td.ntiles), dim3(td.threads), lds, st, p, m.ctrl, m.ring, td, m.dmi,
-                       m.vol, m.fown, m.hpart, m.fownh, m.hparth);
+#define PBS_TILE_LAUNCH(MAT, F32M)                                                                                \
+  hipLaunchKernelGGL((k_tile<R, MAT, F32M>), dim3(td.ntiles), dim3(td.threads), lds, st, p, m.ctrl, m.ring, td, \
+                     m.rest, m.fown, m.hpart, m.fownh, m.hparth)
+  if (p.material == 0 && !f32m) PBS_TILE_LAUNCH(0, false);
+  else if (p.material == 0) PBS_TILE_LAUNCH(0, true);
+  else if (!f32m) PBS_TILE_LAUNCH(1, false);
+  else PBS_TILE_LAUNCH(1, true);
+#undef PBS_TILE_LAUNCH
 }
 template <typename R>
 void Launch<R>::node_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {

@@ -63,6 +63,36 @@ def test_stiffness_matrix_matches_oracle(precision, material):
     s.close()
 
 
+@pytest.mark.parametrize("material", [P.MAT_NEOHOOKEAN, P.MAT_STRETCH])
+def test_stiffness_matrix_wide_rows_fp64(material):
+    """The reference's TetGen bar (tests/golden/model_bar/bar.1) has nodes in up to 28 block
+    columns; in FP64 its widest 64-row slice needs 28 * 9 * 64 * 8 B = 129 KB, beyond the 96 KB of
+    LDS the assembly accumulates rows in, so the matrix is accumulated in place (the other
+    assembly path).  Both materials against the oracle, with a mild random deformation."""
+    import os
+
+    nodes, tets = P.tetgen_load(os.path.join(M.GOLDEN, "model_bar/bar.1"), 0.1)
+    mask = (nodes[:, 0] == nodes[:, 0].min()).astype(np.int32)
+    rng = np.random.default_rng(3)
+    x = (nodes + rng.uniform(-1e-3, 1e-3, nodes.shape)).astype(np.float32)
+    rowptr, col, _, _ = P.csr_pattern(len(nodes), tets)
+    width = np.diff(rowptr)[::3] // 3  # block columns per node (row 3a holds 3 per block)
+    say("bar.1 FP64 stiffness, material", material, "widest row", int(width.max()), "block columns")
+    assert width.max() * 9 * 64 * 8 > 96 * 1024
+    s = P.Solver(nodes, tets, mask, P.default_params(precision=P.FP64, material=material))
+    s.set_positions(x)
+    rowptr, col, val = s.stiffness_matrix(tets)
+    s.close()
+    mu, lam = O.lame(1e5, 0.45)
+    pre = O.precompute(nodes, tets, mask)
+    r2, c2, kval, _, _ = O.implicit_system(nodes, tets, mask, pre, mu, lam, x.astype(np.float64),
+                                           np.zeros_like(x, np.float64), H, material=material)
+    assert np.array_equal(rowptr, r2) and np.array_equal(col, c2)
+    err = np.abs(val - kval).max() / np.abs(kval).max()
+    say("max |K_gpu - K_oracle| / max |K|", err)
+    assert err <= 1e-6
+
+
 @pytest.mark.parametrize("precision", [P.FP32, P.FP64])
 def test_one_step_matches_direct_solve(precision):
     rest, tets, mask, x0, v0 = case()
