@@ -9,7 +9,7 @@ network is trained on the GPU on another seeded trajectory of the same bar (and 
 12x12x50 bar, 43,200 tets, for the second comparison) and then steps the benchmark trajectory.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload bar64x64x256|bar64x64x256-stretch|bar12x12x50|bar4x4x16|batch256] [--mode relgrad|fixed]
-                  [--no-learned] [--no-cpu-baseline]
+                  [--no-learned] [--no-cpu-baseline] [--no-implicit] [--no-batch] [--tile-tets N]
 """
 import argparse
 import json
@@ -29,7 +29,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 # (kernel, bytes per position component) -> bytes.  These are not measured by this script (a
 # counter run serialises every dispatch); the bench line names the profile they come from.
 TRAFFIC_PROFILE = "profiles/k_tile_k_node_bar64x64x256_relgrad_f64.md"
-MEASURED_TRAFFIC = {("k_tile", 8): 476.6e6, ("k_tile", 4): 427.3e6, ("k_node", 8): 202.3e6, ("k_node", 4): 118.5e6}
+MEASURED_TRAFFIC = {("k_tile", 8): 237.6e6, ("k_tile", 4): 188.7e6, ("k_node", 8): 202.4e6, ("k_node", 4): 118.5e6}
 
 WORKLOADS = {
     # name: (nx, ny, nz, material, twist_deg, jitter, fixed iterations)
@@ -61,9 +61,11 @@ def batch_instances(P, count=256, seed=3):
 
 def algorithmic_bytes(nt, nn, rsz):
     """Compulsory HBM bytes per launch of the two kernels of a descent pass (rsz = 4 or 8 bytes
-    per position component).
+    per position component), as the reference's algorithm needs them.
     k_tile: per tet its 4 node indices (int32) and rest data (9 FP32 Dm^-1 entries + FP32 volume)
-            = 56 B; per node the position read and the force written (2 x 3 rsz).
+            = 56 B; per node the position read and the force written (2 x 3 rsz).  (This tile pass
+            reads 16 B per tet and recomputes the rest data from the rest positions, so its
+            measured traffic is below this figure.)
     k_node: per node the reads of x, x~, x_last and the force (4 x 3 rsz), Hessian diagonal
             (3 x FP32), mass (FP32) and mask (int32), and the write of x_next (3 rsz)."""
     t = 56.0 * nt + 6.0 * rsz * nn
@@ -239,6 +241,28 @@ def timed_steps(P, rest, tets, mask, x0, params, warmup, steps, model=None):
             "converged_steps": int(sum(q.converged for q in st)), "rollbacks": int(sum(q.rollbacks for q in st))}
 
 
+def batch_report(P, grav, warmup=1, steps=3):
+    """The 256-bar batch (batch_instances) in one batched solver at 96 FP32 descent iterations per
+    step, tile size picked by the autotuner: steps/s, time per descent pass (the kernels of all
+    instances), and the autotuner's timings."""
+    inst = batch_instances(P)
+    prm = P.default_params(dt=1.0 / 30.0, grav=grav, material=0, iterations=96,
+                           profile=[(1, 0.96), (7, 0.991), (10, 0.9995)], tile_tets=-1)
+    s = P.Solver.batch([(i[0], i[1], i[2]) for i in inst], [i[4] for i in inst], [i[5] for i in inst], prm)
+    s.set_positions(np.concatenate([i[3] for i in inst]))
+    for _ in range(warmup):
+        s.step()
+    t0 = time.perf_counter()
+    st = [s.step() for _ in range(steps)]
+    el = time.perf_counter() - t0
+    out = {"steps_per_s": steps / el, "us_per_pass": 1e6 * el / max(sum(q.passes for q in st), 1),
+           "instances": len(inst), "tets": int(sum(len(i[1]) for i in inst)),
+           "nodes": int(sum(len(i[0]) for i in inst)), "tile_tets": s.get_params().tile_tets,
+           "autotune_us_per_pass": s.autotune_report(), "steps": steps, "warmup": warmup}
+    s.close()
+    return out
+
+
 def warm_start_report(P, args, params, rest, tets, mask, x0, dims, plain_sps, plain_its, other=None):
     """Learned warm start vs plain with one network trained per configuration 4 (train_config4):
     on the benchmark bar (the timed steps' own protocol: warm-up, then the timed steps) and on the
@@ -405,6 +429,9 @@ def main():
                     help="also time one full FP64 rel-grad step of the CPU path (minutes)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
     ap.add_argument("--no-implicit", action="store_true", help="skip the backward-Euler report")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-bar batch report of the default run")
+    ap.add_argument("--tile-tets", type=int, default=None,
+                    help="tile size of the measured solver (default: from the mesh size; the autotuner for batch256)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) for counter calibration")
     args = ap.parse_args()
@@ -458,7 +485,8 @@ def main():
     f64 = args.precision == "f64" or (args.precision == "auto" and relg)
     pkw = dict(dt=1.0 / 30.0, grav=grav, material=material, precision=P.FP64 if f64 else P.FP32,
                iterations=fixed_iters, relgrad_tol=args.tol if relg else 0.0, max_iterations=args.max_iterations,
-               profile=[(1, 0.96), (7, 0.991), (10, 0.9995)], tile_tets=-1 if batch else 0)
+               profile=[(1, 0.96), (7, 0.991), (10, 0.9995)],
+               tile_tets=args.tile_tets if args.tile_tets is not None else (-1 if batch else 0))
     params = P.default_params(**pkw, mixed_switch=args.mixed_switch if (relg and f64) else 0.0)
     if batch:
         s = P.Solver.batch([(i[0], i[1], i[2]) for i in inst], [i[4] for i in inst], [i[5] for i in inst], params)
@@ -532,7 +560,7 @@ def main():
         "learned_warm_start": None,
     }
     if batch:
-        out["autotune_us_per_pass"] = s.autotune_report()
+        out["autotune_us_per_pass"] = s.autotune_report() if params.tile_tets < 0 else None
         out["tile_tets"] = s.get_params().tile_tets
     s.close()
     if relg and f64 and not batch:
@@ -555,6 +583,8 @@ def main():
         out["learned_guess_used_fraction"] = lw["bench_bar"]["learned"]["guess_used_fraction"]
     if not batch and args.workload == "bar64x64x256":
         out["fixed_64_iterations_f32"] = fixed_iteration_report(P, rest, tets, mask, grav)
+        if not args.no_batch:
+            out["batch256"] = batch_report(P, grav)
     if not args.no_implicit and not batch:
         out["implicit_euler"] = implicit_report(P, rest, tets, mask, material, grav)
         out["projective_dynamics"] = pd_report(P, rest, tets, mask, grav)

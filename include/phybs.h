@@ -121,14 +121,27 @@ int pbs_create(int num_nodes, const float *nodes, const float *rest, const int *
 int pbs_create_batch(int num_instances, const int *inst_nodes, const int *inst_tets, const float *nodes,
                      const float *rest, const int *constraint_mask, const int *tets, const float *youngs,
                      const float *nu, const pbs_params *p, pbs_solver **out);
+/* pbs_create_batch with a tile size per instance (inst_tile_tets[i] tets per tile for instance
+ * i; p->tile_tets is not used).  Results equal those of separate solvers, each with its
+ * instance's tile size.  No reference counterpart. */
+int pbs_create_batch_tiled(int num_instances, const int *inst_nodes, const int *inst_tets, const float *nodes,
+                           const float *rest, const int *constraint_mask, const int *tets, const float *youngs,
+                           const float *nu, const int *inst_tile_tets, const pbs_params *p, pbs_solver **out);
 int pbs_num_instances(const pbs_solver *s);
 /* Loop statistics of one instance in the last pbs_step (gpu_ms and g_learned are 0).  pbs_step
  * itself reports, for a batch, passes = the most any instance ran, iterations / rollbacks /
  * converged summed, alpha the smallest, gradient norms combined as sqrt(sum of squares). */
 int pbs_instance_stats(const pbs_solver *s, int instance, pbs_step_stats *st);
 /* Tile sizes timed by the autotuner (pbs_params.tile_tets = -1) and their time per descent pass
- * in microseconds; *count = 0 when the solver was not autotuned. */
+ * in microseconds; *count = 0 when the solver was not autotuned.  For a batch of at least 8
+ * instances the autotuner also tunes each class of instances (4 classes of equal count by tet
+ * count) alone; an entry with tile_tets = -2 is then the batch with per-instance sizes (present
+ * when the classes chose different sizes, or one size other than the fastest uniform one).  No
+ * counterpart in the reference (it has no tile pass). */
 int pbs_autotune_report(const pbs_solver *s, int max, int *tile_tets, double *us_per_pass, int *count);
+/* Tile size of each instance (pbs_num_instances values); after autotuning, a batch's instances
+ * may have different sizes.  No counterpart in the reference. */
+int pbs_instance_tile_tets(const pbs_solver *s, int *tile_tets);
 /* ~CUDAInterface (CUDAInterface.cu:742-769). */
 void pbs_destroy(pbs_solver *s);
 /* Change solver parameters (material constants and gravity re-run the setup kernels). */

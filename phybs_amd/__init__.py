@@ -101,9 +101,11 @@ SIGNATURES = {
     "pbs_abi_version": (C.c_int, []),
     "pbs_create": (C.c_int, [C.c_int, _P, _P, _P, C.c_int, _P, _P, C.POINTER(_P)]),
     "pbs_create_batch": (C.c_int, [C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(_P)]),
+    "pbs_create_batch_tiled": (C.c_int, [C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(_P)]),
     "pbs_num_instances": (C.c_int, [_P]),
     "pbs_instance_stats": (C.c_int, [_P, C.c_int, _P]),
     "pbs_autotune_report": (C.c_int, [_P, C.c_int, _P, _P, _P]),
+    "pbs_instance_tile_tets": (C.c_int, [_P, _P]),
     "pbs_destroy": (None, [_P]),
     "pbs_set_params": (C.c_int, [_P, _P]),
     "pbs_get_params": (C.c_int, [_P, _P]),
@@ -317,10 +319,11 @@ class Solver:
         self.h = h
 
     @classmethod
-    def batch(cls, meshes, youngs, nu, params=None, nodes=None):
+    def batch(cls, meshes, youngs, nu, params=None, nodes=None, tile_tets=None):
         """Batched solver over independent meshes: meshes = [(rest, tets, mask), ...] with tets
         indexing each mesh's own nodes; youngs / nu per mesh.  Arrays of the solver (positions,
-        velocities, ...) are the meshes' arrays concatenated in order."""
+        velocities, ...) are the meshes' arrays concatenated in order.  tile_tets: optional tile
+        size per mesh (else params.tile_tets for all, -1 = autotuned)."""
         self = cls.__new__(cls)
         rests, tl, masks, nns, nts = [], [], [], [], []
         off = 0
@@ -345,8 +348,17 @@ class Solver:
         v = np.ascontiguousarray(nu, np.float32)
         x0 = None if nodes is None else _f32(nodes, (-1, 3))
         h = C.c_void_p()
-        _check(lib().pbs_create_batch(len(meshes), _p(self.inst_nodes), _p(self.inst_tets), _p(x0), _p(self.rest),
-                                      _p(self.mask), _p(self.tets), _p(E), _p(v), C.byref(self.params), C.byref(h)))
+        if tile_tets is None:
+            _check(lib().pbs_create_batch(len(meshes), _p(self.inst_nodes), _p(self.inst_tets), _p(x0),
+                                          _p(self.rest), _p(self.mask), _p(self.tets), _p(E), _p(v),
+                                          C.byref(self.params), C.byref(h)))
+        else:
+            tt = np.ascontiguousarray(tile_tets, np.int32)
+            if tt.shape != (len(meshes),):
+                raise ValueError("tile_tets: one tile size per mesh")
+            _check(lib().pbs_create_batch_tiled(len(meshes), _p(self.inst_nodes), _p(self.inst_tets), _p(x0),
+                                                _p(self.rest), _p(self.mask), _p(self.tets), _p(E), _p(v), _p(tt),
+                                                C.byref(self.params), C.byref(h)))
         self.h = h
         return self
 
@@ -361,7 +373,13 @@ class Solver:
         us = np.zeros(n)
         c = C.c_int()
         _check(lib().pbs_autotune_report(self.h, n, _p(tiles), _p(us), C.byref(c)))
-        return {int(t): float(u) for t, u in zip(tiles[:c.value], us[:c.value])}
+        return {(int(t) if t > 0 else "per_instance"): float(u) for t, u in zip(tiles[:c.value], us[:c.value])}
+
+    def instance_tile_tets(self):
+        """Tile size of each instance (the autotuner may give a batch's instances different sizes)."""
+        out = np.zeros(lib().pbs_num_instances(self.h), np.int32)
+        _check(lib().pbs_instance_tile_tets(self.h, _p(out)))
+        return out
 
     def close(self):
         if getattr(self, "h", None):

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <numeric>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -372,9 +373,11 @@ template <typename R> static int set_positions_impl(pbs_solver *s, const float *
 
 // Creates a solver for ninst independent meshes (1 for pbs_create): instance i owns nodes
 // [node_off[i], node_off[i+1]) and tets [tet_off[i], tet_off[i+1]), tets hold global node ids.
+// tile_tets: one tile size for all instances (0: chosen from the mesh size); inst_tiles, when
+// not null, one per instance instead.
 static int create_impl(int ninst, const int *node_off, const int *tet_off, const float *nodes, const float *rest,
                        const int *mask, const int *tets, const float *youngs, const float *nu, const pbs_params *p,
-                       int tile_tets, pbs_solver **out) {
+                       int tile_tets, const int *inst_tiles, pbs_solver **out) {
   const int nn = node_off[ninst], nt = tet_off[ninst];
   pbs_solver *s = new pbs_solver();
   s->nn = nn;
@@ -412,7 +415,10 @@ static int create_impl(int ninst, const int *node_off, const int *tet_off, const
     while (tt > 64 && (nt / tt < 1024 || lds_need(std::max(64, tt / 2), tt) > std::min<size_t>(s->lds_budget, 40960))) tt /= 2;
   }
   std::string msg;
-  if (!build_batch_plan(ninst, node_off, tet_off, rest, tets, tt, s->plan, msg)) return fail(set_error(PBS_ERR_MESH, msg));
+  const bool planned = inst_tiles ? build_batch_plan(ninst, node_off, tet_off, rest, tets, inst_tiles, s->plan, msg)
+                                  : build_batch_plan(ninst, node_off, tet_off, rest, tets, tt, s->plan, msg);
+  if (!planned) return fail(set_error(PBS_ERR_MESH, msg));
+  if (inst_tiles) tt = s->plan.tile_tets;
   if (lds_need(s->plan.lcap, s->plan.tcap) > s->lds_budget)
     return fail(set_error(PBS_ERR_ARG, "tile too large for the LDS of one workgroup: use a smaller tile_tets"));
   s->p.tile_tets = tt;
@@ -441,8 +447,34 @@ static int create_impl(int ninst, const int *node_off, const int *tet_off, const
   return PBS_OK;
 }
 
-// Autotuner: builds the solver with each candidate tile size, times 32 fixed descent passes from
-// the initial state (after 8 untimed ones) and keeps the fastest.
+// Time of one descent pass (microseconds) of a solver built with the given tile sizes: 8
+// untimed passes from the initial state, then 32 timed ones.  Negative when the solver cannot be
+// built with them (e.g. a tile exceeds the LDS).
+static double time_tiles(int ninst, const int *node_off, const int *tet_off, const float *nodes, const float *rest,
+                         const int *mask, const int *tets, const float *youngs, const float *nu, const pbs_params *p,
+                         int tt, const int *inst_tiles) {
+  pbs_solver *s = nullptr;
+  pbs_params q = *p;
+  q.iterations = 8;
+  q.relgrad_tol = 0.0;
+  if (create_impl(ninst, node_off, tet_off, nodes, rest, mask, tets, youngs, nu, &q, tt, inst_tiles, &s)) return -1.0;
+  pbs_step_stats st{};
+  int rc = pbs_step(s, nullptr, nullptr, &st);
+  q.iterations = 32;
+  if (!rc) rc = pbs_set_params(s, &q);
+  if (!rc) rc = pbs_reset(s);
+  if (!rc && nodes) rc = pbs_set_positions(s, nodes);
+  if (!rc) rc = pbs_step(s, nullptr, nullptr, &st);
+  pbs_destroy(s);
+  if (rc || st.passes <= 0) return -1.0;
+  return 1e3 * st.gpu_ms / st.passes;
+}
+
+// Autotuner: times each candidate tile size on the whole mesh (or batch) and keeps the fastest.
+// A batch of at least 8 instances is also tuned per instance: the instances are split by tet
+// count into 4 classes of equal count, each class alone (a batch of its instances) is timed at
+// every candidate size, and the batch with each instance at its class's fastest size is timed
+// against the fastest uniform size; the faster of the two is built.
 static int create_tuned(int ninst, const int *node_off, const int *tet_off, const float *nodes, const float *rest,
                         const int *mask, const int *tets, const float *youngs, const float *nu, const pbs_params *p,
                         pbs_solver **out) {
@@ -452,21 +484,8 @@ static int create_tuned(int ninst, const int *node_off, const int *tet_off, cons
   int best = -1;
   double best_us = 0.0;
   for (int tt : cand) {
-    pbs_solver *s = nullptr;
-    pbs_params q = *p;
-    q.iterations = 8;
-    q.relgrad_tol = 0.0;
-    if (create_impl(ninst, node_off, tet_off, nodes, rest, mask, tets, youngs, nu, &q, tt, &s)) continue;
-    pbs_step_stats st{};
-    int rc = pbs_step(s, nullptr, nullptr, &st);
-    q.iterations = 32;
-    if (!rc) rc = pbs_set_params(s, &q);
-    if (!rc) rc = pbs_reset(s);
-    if (!rc && nodes) rc = pbs_set_positions(s, nodes);
-    if (!rc) rc = pbs_step(s, nullptr, nullptr, &st);
-    pbs_destroy(s);
-    if (rc || st.passes <= 0) continue;
-    const double t = 1e3 * st.gpu_ms / st.passes;
+    const double t = time_tiles(ninst, node_off, tet_off, nodes, rest, mask, tets, youngs, nu, p, tt, nullptr);
+    if (t < 0) continue;
     tiles.push_back(tt);
     us.push_back(t);
     if (best < 0 || t < best_us) {
@@ -475,10 +494,63 @@ static int create_tuned(int ninst, const int *node_off, const int *tet_off, cons
     }
   }
   if (best < 0) return set_error(PBS_ERR_ARG, "autotuner: no tile size fits this mesh");
-  int rc = create_impl(ninst, node_off, tet_off, nodes, rest, mask, tets, youngs, nu, p, best, out);
+  std::vector<int> inst_tile;
+  std::vector<int> class_tile;
+  if (ninst >= 8) {
+    constexpr int K = 4;
+    std::vector<int> order(ninst);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      return tet_off[a + 1] - tet_off[a] < tet_off[b + 1] - tet_off[b];
+    });
+    inst_tile.assign(ninst, best);
+    for (int c = 0; c < K; ++c) {
+      std::vector<int> mem(order.begin() + (size_t)c * ninst / K, order.begin() + (size_t)(c + 1) * ninst / K);
+      std::sort(mem.begin(), mem.end());
+      // the class as a batch of its own: instance arrays copied, node ids re-based
+      std::vector<int> no(1, 0), to(1, 0), msk, tt4;
+      std::vector<float> nd, rs, E, V;
+      for (int i : mem) {
+        const int n0 = node_off[i], n1 = node_off[i + 1], t0 = tet_off[i], t1 = tet_off[i + 1];
+        for (long k = 4L * t0; k < 4L * t1; ++k) tt4.push_back(tets[k] - n0 + no.back());
+        rs.insert(rs.end(), rest + 3L * n0, rest + 3L * n1);
+        if (nodes) nd.insert(nd.end(), nodes + 3L * n0, nodes + 3L * n1);
+        if (mask) msk.insert(msk.end(), mask + n0, mask + n1);
+        E.push_back(youngs[i]);
+        V.push_back(nu[i]);
+        no.push_back(no.back() + n1 - n0);
+        to.push_back(to.back() + t1 - t0);
+      }
+      int cbest = best;
+      double cbest_us = -1.0;
+      for (int tt : cand) {
+        const double t = time_tiles((int)mem.size(), no.data(), to.data(), nodes ? nd.data() : nullptr, rs.data(),
+                                    mask ? msk.data() : nullptr, tt4.data(), E.data(), V.data(), p, tt, nullptr);
+        if (t >= 0 && (cbest_us < 0 || t < cbest_us)) {
+          cbest = tt;
+          cbest_us = t;
+        }
+      }
+      class_tile.push_back(cbest);
+      for (int i : mem) inst_tile[i] = cbest;
+    }
+    if (std::all_of(class_tile.begin(), class_tile.end(), [&](int t) { return t == class_tile[0]; })) {
+      if (class_tile[0] == best) inst_tile.clear();  // the same as the fastest uniform size
+    }
+    if (!inst_tile.empty()) {
+      const double t = time_tiles(ninst, node_off, tet_off, nodes, rest, mask, tets, youngs, nu, p, 0,
+                                  inst_tile.data());
+      tiles.push_back(-2);
+      us.push_back(t);
+      if (!(t >= 0 && t < best_us)) inst_tile.clear();
+    }
+  }
+  int rc = create_impl(ninst, node_off, tet_off, nodes, rest, mask, tets, youngs, nu, p, best,
+                       inst_tile.empty() ? nullptr : inst_tile.data(), out);
   if (rc) return rc;
   (*out)->tune_tiles = tiles;
   (*out)->tune_us = us;
+  (*out)->tune_class_tiles = class_tile;
   return PBS_OK;
 }
 
@@ -486,7 +558,7 @@ static int create_any(int ninst, const int *node_off, const int *tet_off, const 
                       const int *mask, const int *tets, const float *youngs, const float *nu, const pbs_params *p,
                       pbs_solver **out) {
   if (p->tile_tets < 0) return create_tuned(ninst, node_off, tet_off, nodes, rest, mask, tets, youngs, nu, p, out);
-  return create_impl(ninst, node_off, tet_off, nodes, rest, mask, tets, youngs, nu, p, p->tile_tets, out);
+  return create_impl(ninst, node_off, tet_off, nodes, rest, mask, tets, youngs, nu, p, p->tile_tets, nullptr, out);
 }
 
 extern "C" int pbs_create(int nn, const float *nodes, const float *rest, const int *mask, int nt, const int *tets,
@@ -502,9 +574,9 @@ extern "C" int pbs_create(int nn, const float *nodes, const float *rest, const i
   return create_any(1, node_off, tet_off, nodes, rest, mask, tets, &p->youngs, &p->nu, p, out);
 }
 
-extern "C" int pbs_create_batch(int ninst, const int *inst_nodes, const int *inst_tets, const float *nodes,
-                                const float *rest, const int *mask, const int *tets, const float *youngs,
-                                const float *nu, const pbs_params *p, pbs_solver **out) {
+static int create_batch_any(int ninst, const int *inst_nodes, const int *inst_tets, const float *nodes,
+                            const float *rest, const int *mask, const int *tets, const float *youngs, const float *nu,
+                            const int *inst_tiles, const pbs_params *p, pbs_solver **out) {
   if (!out) return set_error(PBS_ERR_ARG, "null out");
   *out = nullptr;
   if (ninst <= 0 || !inst_nodes || !inst_tets || !rest || !youngs || !nu)
@@ -525,7 +597,27 @@ extern "C" int pbs_create_batch(int ninst, const int *inst_nodes, const int *ins
     for (long k = 4L * tet_off[i]; k < 4L * tet_off[i + 1]; ++k)
       if (tets[k] < node_off[i] || tets[k] >= node_off[i + 1])
         return set_error(PBS_ERR_MESH, "tet index outside its instance's node range");
+  if (inst_tiles) {
+    for (int i = 0; i < ninst; ++i)
+      if (inst_tiles[i] < 1 || inst_tiles[i] > 16383) return set_error(PBS_ERR_ARG, "instance tile size out of range");
+    return create_impl(ninst, node_off.data(), tet_off.data(), nodes, rest, mask, tets, youngs, nu, p, 0, inst_tiles,
+                       out);
+  }
   return create_any(ninst, node_off.data(), tet_off.data(), nodes, rest, mask, tets, youngs, nu, p, out);
+}
+
+extern "C" int pbs_create_batch(int ninst, const int *inst_nodes, const int *inst_tets, const float *nodes,
+                                const float *rest, const int *mask, const int *tets, const float *youngs,
+                                const float *nu, const pbs_params *p, pbs_solver **out) {
+  return create_batch_any(ninst, inst_nodes, inst_tets, nodes, rest, mask, tets, youngs, nu, nullptr, p, out);
+}
+
+extern "C" int pbs_create_batch_tiled(int ninst, const int *inst_nodes, const int *inst_tets, const float *nodes,
+                                      const float *rest, const int *mask, const int *tets, const float *youngs,
+                                      const float *nu, const int *inst_tile_tets, const pbs_params *p,
+                                      pbs_solver **out) {
+  if (!inst_tile_tets) return set_error(PBS_ERR_ARG, "null inst_tile_tets");
+  return create_batch_any(ninst, inst_nodes, inst_tets, nodes, rest, mask, tets, youngs, nu, inst_tile_tets, p, out);
 }
 
 extern "C" int pbs_num_instances(const pbs_solver *s) {
@@ -547,6 +639,12 @@ extern "C" int pbs_instance_stats(const pbs_solver *s, int inst, pbs_step_stats 
   st->energy = c.E_old;
   st->g0_norm = c.g0;
   st->g_norm = c.g;
+  return PBS_OK;
+}
+
+extern "C" int pbs_instance_tile_tets(const pbs_solver *s, int *tile_tets) {
+  if (!s || !tile_tets) return set_error(PBS_ERR_ARG, "null argument");
+  for (int i = 0; i < s->ninst; ++i) tile_tets[i] = s->plan.inst_tile_tets[i];
   return PBS_OK;
 }
 

@@ -41,7 +41,8 @@ struct pbs_solver {
   std::vector<float> inst_youngs, inst_nu;
   std::vector<Ctrl> ctrl_h;               // last copy of the control blocks
   std::vector<int> tune_tiles;            // autotuner: candidate tile sizes ...
-  std::vector<double> tune_us;            // ... and their time per descent pass
+  std::vector<double> tune_us;            // ... and their time per descent pass (tile -2: per-instance sizes)
+  std::vector<int> tune_class_tiles;      // autotuner, batches: fastest size of each instance class
   pbs_params p{};
   DevParams dp{};
   int bs = 256, ntb = 0, nnb = 0;

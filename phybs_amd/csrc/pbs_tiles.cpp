@@ -213,11 +213,18 @@ bool build_tile_plan(int nn, const float *rest, int nt, const int *tets, int til
 
 bool build_batch_plan(int ninst, const int *node_off, const int *tet_off, const float *rest, const int *tets,
                       int tile_tets, TilePlan &P, std::string &msg) {
+  const std::vector<int> tt(std::max(ninst, 0), tile_tets);
+  return build_batch_plan(ninst, node_off, tet_off, rest, tets, tt.data(), P, msg);
+}
+
+bool build_batch_plan(int ninst, const int *node_off, const int *tet_off, const float *rest, const int *tets,
+                      const int *inst_tile_tets, TilePlan &P, std::string &msg) {
   P = TilePlan();
   P.ninst = ninst;
   P.nn = node_off[ninst];
   P.nt = tet_off[ninst];
-  P.tile_tets = tile_tets;
+  P.inst_tile_tets.assign(inst_tile_tets, inst_tile_tets + ninst);
+  P.tile_tets = ninst > 0 ? *std::max_element(inst_tile_tets, inst_tile_tets + ninst) : 0;
   P.inst_node_off.assign(node_off, node_off + ninst + 1);
   P.inst_tet_off.assign(tet_off, tet_off + ninst + 1);
   P.inst_tile_off.assign(1, 0);
@@ -239,7 +246,7 @@ bool build_batch_plan(int ninst, const int *node_off, const int *tet_off, const 
       lt[k] = g - n0;
     }
     TilePlan Q;
-    if (!build_tile_plan(n1 - n0, rest + 3 * (size_t)n0, t1 - t0, lt.data(), tile_tets, Q, msg)) {
+    if (!build_tile_plan(n1 - n0, rest + 3 * (size_t)n0, t1 - t0, lt.data(), inst_tile_tets[i], Q, msg)) {
       msg = "instance " + std::to_string(i) + ": " + msg;
       return false;
     }

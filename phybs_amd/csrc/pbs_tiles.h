@@ -35,6 +35,7 @@ struct TilePlan {
   int ninst = 1;
   std::vector<int> inst_node_off, inst_tet_off, inst_tile_off;
   int tile_tets = 0, ntiles = 0;
+  std::vector<int> inst_tile_tets;  // per instance: its tile size
   int lcap = 0;                     // max local nodes (owned + halo) over tiles
   int tcap = 0;                     // max tets over tiles
   long halo_total = 0;              // sum of halo counts = number of halo partial slots
@@ -57,7 +58,11 @@ bool build_tile_plan(int nn, const float *rest, int nt, const int *tets, int til
                      std::string &msg);
 // Batched: ninst meshes, instance i owning nodes [node_off[i], node_off[i+1]) and tets
 // [tet_off[i], tet_off[i+1]) of the concatenated arrays (tets hold global node ids inside their
-// instance's range).  Each instance is planned on its own; the plans are concatenated.
+// instance's range).  Each instance is planned on its own with its tile size inst_tile_tets[i];
+// the plans are concatenated (plan.tile_tets = the largest of them).
+bool build_batch_plan(int ninst, const int *node_off, const int *tet_off, const float *rest, const int *tets,
+                      const int *inst_tile_tets, TilePlan &plan, std::string &msg);
+// The same with one tile size for every instance.
 bool build_batch_plan(int ninst, const int *node_off, const int *tet_off, const float *rest, const int *tets,
                       int tile_tets, TilePlan &plan, std::string &msg);
 

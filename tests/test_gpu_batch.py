@@ -159,3 +159,82 @@ def test_autotuner_picks_a_timed_tile_size():
         t.step()
     assert np.array_equal(t.positions().view(np.uint32), x_tuned.view(np.uint32))
     t.close()
+
+
+@pytest.mark.parametrize("mode", ["fixed_f32", "relgrad_f64"])
+def test_batch_with_per_instance_tiles_equals_separate_solvers(mode):
+    """Each instance at its own tile size (16 to 512 tets per tile) in one batch: each instance's
+    trajectory and loop statistics are bitwise those of its own solver at that tile size."""
+    inst = instances(seed=4, k=6)
+    tiles = [16, 64, 512, 128, 32, 256]
+    kw = dict(grav=(0.0, 0.0, -9.8))
+    if mode == "relgrad_f64":
+        kw.update(precision=P.FP64, relgrad_tol=1e-4, max_iterations=5000)
+    say("per-instance tiles", mode, tiles, [len(t) for _, t, _, _, _, _ in inst])
+    sep_x, sep_st = [], []
+    for (rest, tets, mask, x0, E, nu), tt in zip(inst, tiles):
+        s = P.Solver(rest, tets, mask, P.default_params(youngs=E, nu=nu, tile_tets=tt, **kw))
+        s.set_positions(x0)
+        st = [s.step() for _ in range(3)]
+        sep_x.append(s.positions(f64=True))
+        sep_st.append([(q.iterations, q.rollbacks, q.passes) for q in st])
+        s.close()
+    b = P.Solver.batch([(r, t, m) for r, t, m, _, _, _ in inst], [i[4] for i in inst], [i[5] for i in inst],
+                       P.default_params(**kw), tile_tets=tiles)
+    assert list(b.instance_tile_tets()) == tiles
+    b.set_positions(np.concatenate([i[3] for i in inst]))
+    per = [[] for _ in inst]
+    for _ in range(3):
+        b.step()
+        for i in range(len(inst)):
+            q = b.instance_stats(i)
+            per[i].append((q.iterations, q.rollbacks, q.passes))
+    xb = b.positions(f64=True)
+    b.close()
+    off = 0
+    for i, x in enumerate(sep_x):
+        assert per[i] == sep_st[i], (i, per[i], sep_st[i])
+        assert np.array_equal(xb[off:off + len(x)].view(np.uint64), x.view(np.uint64)), i
+        off += len(x)
+
+
+def test_autotuner_per_instance_tile_sizes():
+    """A batch of 12 bars of 96 to 3,456 tets autotuned (tile_tets = -1): every uniform candidate
+    and, when the size classes disagree, the per-instance mix are timed; instances of one class
+    share a size; the kept configuration is the fastest timed; the tuned batch steps bitwise like
+    a batch built with the same per-instance sizes."""
+    rng = np.random.default_rng(21)
+    meshes, x0s, Es, nus = [], [], [], []
+    for i in range(12):
+        nx, ny, nz = int(rng.integers(2, 7)), int(rng.integers(2, 7)), int(rng.integers(4, 17))
+        rest, tets = M.bar(nx, ny, nz, 0.05)
+        meshes.append((rest, tets, M.clamp_x0(rest)))
+        x0s.append(M.twisted_state(rest, nx, ny, nz, 0.05, 20.0, 0.01, 30 + i))
+        Es.append(float(np.exp(rng.uniform(np.log(1e4), np.log(1e6)))))
+        nus.append(float(rng.uniform(0.3, 0.49)))
+    x0 = np.concatenate(x0s)
+    kw = dict(grav=(0.0, 0.0, -9.8))
+    s = P.Solver.batch(meshes, Es, nus, P.default_params(tile_tets=-1, **kw), nodes=x0)
+    rep = s.autotune_report()
+    tt = s.instance_tile_tets()
+    say("autotune", rep, "per instance", list(tt), "tets", [len(m[1]) for m in meshes])
+    uniform = {k: v for k, v in rep.items() if k != "per_instance"}
+    assert len(uniform) >= 2 and all(k in (64, 128, 256, 512) for k in uniform)
+    assert all(t in (64, 128, 256, 512) for t in tt)
+    order = np.argsort([len(m[1]) for m in meshes], kind="stable")
+    for c in range(4):
+        assert len(set(tt[order[3 * c:3 * c + 3]])) == 1
+    best_uniform = min(uniform, key=uniform.get)
+    if "per_instance" in rep and rep["per_instance"] < uniform[best_uniform]:
+        say("kept: per-instance sizes")
+    else:
+        assert all(t == best_uniform for t in tt)
+    for _ in range(2):
+        s.step()
+    x_tuned = s.positions(f64=True)
+    s.close()
+    b = P.Solver.batch(meshes, Es, nus, P.default_params(**kw), nodes=x0, tile_tets=list(tt))
+    for _ in range(2):
+        b.step()
+    assert np.array_equal(b.positions(f64=True).view(np.uint64), x_tuned.view(np.uint64))
+    b.close()
