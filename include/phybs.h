@@ -164,6 +164,14 @@ int pbs_get_positions(pbs_solver *s, float *xyz);
 int pbs_get_positions_f64(pbs_solver *s, double *xyz);
 int pbs_set_velocities(pbs_solver *s, const float *v, const float *v_last);
 int pbs_get_velocities(pbs_solver *s, float *v, float *v_last);
+/* Binary snapshot of the state a step starts from -- positions, velocities and previous
+ * velocities (x, v, v_last) at the solver's precision -- and its restore into a solver built on
+ * the same meshes (same node counts per instance and precision; otherwise PBS_ERR_ARG and the
+ * state is left unchanged).  The reference checkpoints positions only, as a .node file
+ * (TetMesh.cpp:558-570, restored through CUDAInterface::updateNodePositions, .cu:771-774); a
+ * restored FP64 solver continues bitwise as the saved one would have. */
+int pbs_state_save(pbs_solver *s, const char *path);
+int pbs_state_load(pbs_solver *s, const char *path);
 
 /* ---- the hot path ----
  * One implicit-Euler step by Jacobi-preconditioned descent with Chebyshev acceleration and the

@@ -93,6 +93,8 @@ class Solver {
   void positions(float *xyz) const { check(pbs_get_positions(h_, xyz)); }
   void set_velocities(const float *v, const float *v_last) { check(pbs_set_velocities(h_, v, v_last)); }
   void velocities(float *v, float *v_last) const { check(pbs_get_velocities(h_, v, v_last)); }
+  void save_state(const char *path) { check(pbs_state_save(h_, path)); }
+  void load_state(const char *path) { check(pbs_state_load(h_, path)); }
   pbs_step_stats step(const float *ext = nullptr, float *out = nullptr) {
     pbs_step_stats st;
     check(pbs_step(h_, ext, out, &st));

@@ -106,6 +106,8 @@ SIGNATURES = {
     "pbs_instance_stats": (C.c_int, [_P, C.c_int, _P]),
     "pbs_autotune_report": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "pbs_instance_tile_tets": (C.c_int, [_P, _P]),
+    "pbs_state_save": (C.c_int, [_P, C.c_char_p]),
+    "pbs_state_load": (C.c_int, [_P, C.c_char_p]),
     "pbs_destroy": (None, [_P]),
     "pbs_set_params": (C.c_int, [_P, _P]),
     "pbs_get_params": (C.c_int, [_P, _P]),
@@ -374,6 +376,14 @@ class Solver:
         c = C.c_int()
         _check(lib().pbs_autotune_report(self.h, n, _p(tiles), _p(us), C.byref(c)))
         return {(int(t) if t > 0 else "per_instance"): float(u) for t, u in zip(tiles[:c.value], us[:c.value])}
+
+    def save_state(self, path):
+        """Binary snapshot of (x, v, v_last) at the solver's precision."""
+        _check(lib().pbs_state_save(self.h, os.fsencode(path)))
+
+    def load_state(self, path):
+        """Restore a snapshot written by save_state of a solver on the same meshes."""
+        _check(lib().pbs_state_load(self.h, os.fsencode(path)))
 
     def instance_tile_tets(self):
         """Tile size of each instance (the autotuner may give a batch's instances different sizes)."""

@@ -10,6 +10,8 @@
 
 #include <algorithm>
 #include <numeric>
+#include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -755,6 +757,99 @@ template <typename R> static int get_vel_impl(pbs_solver *s, float *v, float *vl
 extern "C" int pbs_get_velocities(pbs_solver *s, float *v, float *vl) {
   if (!s) return set_error(PBS_ERR_ARG, "null handle");
   return s->p.precision == PBS_FP64 ? get_vel_impl<double>(s, v, vl) : get_vel_impl<float>(s, v, vl);
+}
+
+// ---- state snapshot ----
+// File: "PBSSTAT1", int32 precision (4 or 8 bytes per value), int32 node count, int32 instance
+// count, int32 0, int32 node count of each instance, then x, v and v_last, each 3N values AoS in
+// the caller's node order at the solver's precision (so an FP64 run resumes bitwise).
+static const char kStateMagic[8] = {'P', 'B', 'S', 'S', 'T', 'A', 'T', '1'};
+
+template <typename R> static int state_save_impl(pbs_solver *s, FILE *f) {
+  DevMesh<R> &m = mesh<R>(s);
+  const size_t n3 = 3 * (size_t)s->nn;
+  std::vector<R> h(n3);
+  const R *src[3] = {m.xpos, m.v, m.vl};
+  for (const R *planes : src) {
+    if (sizeof(R) == 8) {
+      Launch<R>::planes_to_aos64(planes, s->stage64, m.perm, s->nn, s->st);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(h.data(), s->stage64, sizeof(double) * n3, hipMemcpyDeviceToHost, s->st));
+    } else {
+      Launch<R>::planes_to_aos(planes, s->stage, m.perm, s->nn, s->st);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(h.data(), s->stage, sizeof(float) * n3, hipMemcpyDeviceToHost, s->st));
+    }
+    HIPCHK(hipStreamSynchronize(s->st));
+    if (std::fwrite(h.data(), sizeof(R), n3, f) != n3) return set_error(PBS_ERR_ARG, "state snapshot: write failed");
+  }
+  return PBS_OK;
+}
+
+template <typename R> static int state_load_impl(pbs_solver *s, FILE *f) {
+  DevMesh<R> &m = mesh<R>(s);
+  const size_t n3 = 3 * (size_t)s->nn;
+  std::vector<R> h[3];
+  for (auto &a : h) {  // read everything before touching the device state
+    a.resize(n3);
+    if (std::fread(a.data(), sizeof(R), n3, f) != n3) return set_error(PBS_ERR_ARG, "state snapshot: file too short");
+  }
+  R *dst[3] = {m.xpos, m.v, m.vl};
+  for (int k = 0; k < 3; ++k) {
+    if (sizeof(R) == 8) {
+      HIPCHK(hipMemcpyAsync(s->stage64, h[k].data(), sizeof(double) * n3, hipMemcpyHostToDevice, s->st));
+      Launch<R>::aos64_to_planes(s->stage64, dst[k], m.perm, s->nn, s->st);
+    } else {
+      HIPCHK(hipMemcpyAsync(s->stage, h[k].data(), sizeof(float) * n3, hipMemcpyHostToDevice, s->st));
+      Launch<R>::aos_to_planes(s->stage, dst[k], m.perm, s->nn, s->st);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->st));
+  }
+  return PBS_OK;
+}
+
+extern "C" int pbs_state_save(pbs_solver *s, const char *path) {
+  if (!s || !path) return set_error(PBS_ERR_ARG, "null argument");
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return set_error(PBS_ERR_ARG, std::string("state snapshot: cannot open ") + path);
+  const bool f64 = s->p.precision == PBS_FP64;
+  const int32_t head[4] = {f64 ? 8 : 4, s->nn, s->ninst, 0};
+  std::vector<int32_t> inst(s->ninst);
+  for (int i = 0; i < s->ninst; ++i) inst[i] = s->plan.inst_node_off[i + 1] - s->plan.inst_node_off[i];
+  int rc = PBS_OK;
+  if (std::fwrite(kStateMagic, 1, 8, f) != 8 || std::fwrite(head, sizeof(int32_t), 4, f) != 4 ||
+      std::fwrite(inst.data(), sizeof(int32_t), inst.size(), f) != inst.size())
+    rc = set_error(PBS_ERR_ARG, "state snapshot: write failed");
+  if (!rc) rc = f64 ? state_save_impl<double>(s, f) : state_save_impl<float>(s, f);
+  if (std::fclose(f) != 0 && !rc) rc = set_error(PBS_ERR_ARG, "state snapshot: write failed");
+  return rc;
+}
+
+extern "C" int pbs_state_load(pbs_solver *s, const char *path) {
+  if (!s || !path) return set_error(PBS_ERR_ARG, "null argument");
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return set_error(PBS_ERR_ARG, std::string("state snapshot: cannot open ") + path);
+  const bool f64 = s->p.precision == PBS_FP64;
+  char magic[8];
+  int32_t head[4];
+  int rc = PBS_OK;
+  if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, kStateMagic, 8) != 0 ||
+      std::fread(head, sizeof(int32_t), 4, f) != 4)
+    rc = set_error(PBS_ERR_ARG, "state snapshot: not a snapshot file");
+  else if (head[0] != (f64 ? 8 : 4) || head[1] != s->nn || head[2] != s->ninst)
+    rc = set_error(PBS_ERR_ARG, "state snapshot: precision, node count or instance count differs from this solver's");
+  if (!rc) {
+    std::vector<int32_t> inst(s->ninst);
+    if (std::fread(inst.data(), sizeof(int32_t), inst.size(), f) != inst.size())
+      rc = set_error(PBS_ERR_ARG, "state snapshot: file too short");
+    for (int i = 0; i < s->ninst && !rc; ++i)
+      if (inst[i] != s->plan.inst_node_off[i + 1] - s->plan.inst_node_off[i])
+        rc = set_error(PBS_ERR_ARG, "state snapshot: instance node counts differ from this solver's");
+  }
+  if (!rc) rc = f64 ? state_load_impl<double>(s, f) : state_load_impl<float>(s, f);
+  std::fclose(f);
+  return rc;
 }
 
 // ---- the step ----

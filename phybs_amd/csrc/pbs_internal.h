@@ -124,6 +124,7 @@ template <typename R> struct Launch {
   static void aos_to_planes(const float *aos, R *planes, const int *perm, int n, hipStream_t st);
   static void planes_to_aos(const R *planes, float *aos, const int *perm, int n, hipStream_t st);
   static void planes_to_aos64(const R *planes, double *aos, const int *perm, int n, hipStream_t st);
+  static void aos64_to_planes(const double *aos, R *planes, const int *perm, int n, hipStream_t st);
 };
 // ---- learned warm start (pbs_learn.hip) ----
 // Per-vertex network 18 -> 64 -> 64 -> 64 -> 3, SiLU; packed FP32 parameters W1 (18x64, in x out),

@@ -945,6 +945,10 @@ template <typename R>
 void Launch<R>::planes_to_aos64(const R *planes, double *aos, const int *perm, int n, hipStream_t st) {
   hipLaunchKernelGGL((k_planes_to_aos<R, double>), dim3(nblk(n, 256)), dim3(256), 0, st, planes, aos, perm, n);
 }
+template <typename R>
+void Launch<R>::aos64_to_planes(const double *aos, R *planes, const int *perm, int n, hipStream_t st) {
+  hipLaunchKernelGGL((k_aos_to_planes<double, R>), dim3(nblk(n, 256)), dim3(256), 0, st, aos, planes, perm, n);
+}
 void aos_to_planes_f(const float *aos, float *planes, const int *perm, int n, hipStream_t st) {
   hipLaunchKernelGGL((k_aos_to_planes<float, float>), dim3(nblk(n, 256)), dim3(256), 0, st, aos, planes, perm, n);
 }

@@ -238,3 +238,41 @@ def test_autotuner_per_instance_tile_sizes():
         b.step()
     assert np.array_equal(b.positions(f64=True).view(np.uint64), x_tuned.view(np.uint64))
     b.close()
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_state_snapshot_resumes_a_batch_bitwise(tmp_path, prec):
+    """Save (x, v, v_last) of a batch after 2 steps, step 3 more; a new batch on the same meshes
+    loads the snapshot and steps 3 more: bitwise the same positions, velocities and loop
+    statistics.  A snapshot of other meshes is refused and leaves the state as it was."""
+    inst = instances(seed=6, k=4)
+    kw = dict(tile_tets=64, grav=(0.0, 0.0, -9.8))
+    if prec == "f64":
+        kw.update(precision=P.FP64, relgrad_tol=1e-4, max_iterations=5000)
+    meshes = [(r, t, m) for r, t, m, _, _, _ in inst]
+    Es, nus = [i[4] for i in inst], [i[5] for i in inst]
+    x0 = np.concatenate([i[3] for i in inst])
+    path = str(tmp_path / "state.bin")
+    a = P.Solver.batch(meshes, Es, nus, P.default_params(**kw), nodes=x0)
+    for _ in range(2):
+        a.step()
+    a.save_state(path)
+    sa = [(q.iterations, q.rollbacks, q.passes) for q in (a.step() for _ in range(3))]
+    xa, va = a.positions(f64=True), a.velocities()
+    a.close()
+    say("snapshot", prec, "bytes", (tmp_path / "state.bin").stat().st_size)
+    b = P.Solver.batch(meshes, Es, nus, P.default_params(**kw), nodes=x0)
+    b.load_state(path)
+    sb = [(q.iterations, q.rollbacks, q.passes) for q in (b.step() for _ in range(3))]
+    xb, vb = b.positions(f64=True), b.velocities()
+    assert sa == sb
+    assert np.array_equal(xa.view(np.uint64), xb.view(np.uint64))
+    for u, w in zip(va, vb):
+        assert np.array_equal(u.view(np.uint32), w.view(np.uint32))
+    other = P.Solver.batch(meshes[:3], Es[:3], nus[:3], P.default_params(**kw))
+    other.save_state(str(tmp_path / "other.bin"))
+    other.close()
+    with pytest.raises(RuntimeError, match="snapshot"):
+        b.load_state(str(tmp_path / "other.bin"))
+    assert np.array_equal(b.positions(f64=True).view(np.uint64), xb.view(np.uint64))
+    b.close()
