@@ -9,11 +9,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <numeric>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <numeric>
 #include <string>
 #include <vector>
 
