@@ -9,6 +9,7 @@
 // are a permutation grouped by owner node.
 //
 //   usage: host_check <golden dir>      (exit status 0 = all checks passed)
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -204,20 +205,36 @@ static void check_batch() {
   add(2, 1, 3);
   add(4, 3, 5);
   const int ninst = (int)node_off.size() - 1;
-  pbs::TilePlan P;
-  std::string msg;
-  if (!pbs::build_batch_plan(ninst, node_off.data(), tet_off.data(), nodes.data(), tets.data(), 32, P, msg)) {
-    std::fprintf(stderr, "FAIL batch plan: %s\n", msg.c_str());
-    ++g_fail;
-    return;
-  }
-  // tets_new holds solver node ids; express the mesh in solver order for the check
-  check_plan("batch of 3 instances tile 32", node_off.back(), tet_off.back(), tets, P);
-  for (int b = 0; b < P.ntiles; ++b) {
-    const pbs::TileInfo &ti = P.tiles[b];
-    if (!(ti.inst >= 0 && ti.inst < ninst && ti.own_start >= P.inst_node_off[ti.inst] &&
-          ti.own_start + ti.own_cnt <= P.inst_node_off[ti.inst + 1])) {
-      std::fprintf(stderr, "FAIL batch tile %d mixes instances\n", b);
+  // one tile size for all instances, then one per instance (tiles never exceed their instance's)
+  const std::vector<int> uniform(ninst, 32), mixed = {16, 8, 64};
+  for (const std::vector<int> *tiles : {&uniform, &mixed}) {
+    pbs::TilePlan P;
+    std::string msg;
+    if (!pbs::build_batch_plan(ninst, node_off.data(), tet_off.data(), nodes.data(), tets.data(), tiles->data(), P,
+                               msg)) {
+      std::fprintf(stderr, "FAIL batch plan: %s\n", msg.c_str());
+      ++g_fail;
+      return;
+    }
+    check_plan(tiles == &uniform ? "batch of 3 instances tile 32" : "batch of 3 instances tiles 16/8/64",
+               node_off.back(), tet_off.back(), tets, P);
+    for (int b = 0; b < P.ntiles; ++b) {
+      const pbs::TileInfo &ti = P.tiles[b];
+      if (!(ti.inst >= 0 && ti.inst < ninst && ti.own_start >= P.inst_node_off[ti.inst] &&
+            ti.own_start + ti.own_cnt <= P.inst_node_off[ti.inst + 1])) {
+        std::fprintf(stderr, "FAIL batch tile %d mixes instances\n", b);
+        ++g_fail;
+        return;
+      }
+      if (ti.tet_cnt > (*tiles)[ti.inst]) {
+        std::fprintf(stderr, "FAIL batch tile %d has %d tets, above its instance's tile size %d\n", b, ti.tet_cnt,
+                     (*tiles)[ti.inst]);
+        ++g_fail;
+        return;
+      }
+    }
+    if (P.tile_tets != *std::max_element(tiles->begin(), tiles->end()) || P.inst_tile_tets != *tiles) {
+      std::fprintf(stderr, "FAIL batch plan tile sizes\n");
       ++g_fail;
       return;
     }
