@@ -162,24 +162,25 @@ def collect_trajectory(P, rest, tets, mask, x0, params, steps, seed, stride, cap
     return s
 
 
-def train_config4(P, params, c2_dims, epochs=30, seed=7):
-    """The warm start's training set and network (configuration 4): pairs generated by the plain
-    solver on seeded trajectories of configurations 1-3, none of them the evaluation trajectories
-    (those start from rest without jitter, and from a 90 degree twist with seed 11):
+def train_pooled_network(P, params, bar_dims, epochs=30, seed=7):
+    """The warm start's training set and network: pairs generated by the plain solver on seeded
+    trajectories of the 64x64x256 bar, the 12x12x50 bar and the batch's bars, none of them the
+    evaluation trajectories (those start from rest without jitter, and from a 90 degree twist with
+    seed 11):
       * the 64x64x256 bar from rest under gravity with 0.2 % jitter (seed 7), 30 steps, every 4th
         free node per step;
       * the 12x12x50 bar released from 90 degree twists with 1 % jitter (seeds 2 and 3), 60 steps
         each, every 2nd free node;
-      * 6 bars of the batched configuration's distribution (seed 5) from rest, 40 steps each.
+      * 6 bars of the 256-bar batch's distribution (seed 5) from rest, 40 steps each.
     All with impulses on 20 random nodes every 10th step.  One network (Adam, 30 epochs) on the
     pooled samples, evaluated on both the 64x64x256 and the 12x12x50 bar.  A second network trained
     the same way on the 64x64x256 samples alone (the benchmark bar's own regime, same held-out
     trajectory) is returned beside it: pooling the smaller bars' samples costs the large bar part
     of its gain.  Returns (pooled network, bar network, info)."""
     t0 = time.perf_counter()
-    rest, tets = P.mesh_bar(*c2_dims, 0.05)
+    rest, tets = P.mesh_bar(*bar_dims, 0.05)
     mask = (rest[:, 0] == 0).astype(np.int32)
-    x0 = P.mesh_twist(rest, *c2_dims, 0.05, 0.0, 0.002, seed)
+    x0 = P.mesh_twist(rest, *bar_dims, 0.05, 0.0, 0.002, seed)
     s = collect_trajectory(P, rest, tets, mask, x0, params, 30, seed, 4, 12_000_000)
     counts = {"bar64x64x256": s.learn_samples()}
     s.learn_fit_scales()
@@ -211,7 +212,7 @@ def train_config4(P, params, c2_dims, epochs=30, seed=7):
     loss = s.learn_train(epochs, batch=8192, lr=2e-3, seed=seed)
     t2 = time.perf_counter()
     model = s.learn_get_model()
-    info = {"samples": int(s.learn_samples()), "samples_by_configuration": counts, "epochs": epochs,
+    info = {"samples": int(s.learn_samples()), "samples_by_mesh": counts, "epochs": epochs,
             "loss_first_last": [float(loss[0]), float(loss[-1])],
             "bar_network_loss_first_last": [float(loss_bar[0]), float(loss_bar[-1])],
             "collect_and_bar_network_s": t1c - t0, "train_s": t2 - t1c,
@@ -264,12 +265,12 @@ def batch_report(P, grav, warmup=1, steps=3):
 
 
 def warm_start_report(P, args, params, rest, tets, mask, x0, dims, plain_sps, plain_its, other=None):
-    """Learned warm start vs plain with one network trained per configuration 4 (train_config4):
+    """Learned warm start vs plain with the pooled network (train_pooled_network):
     on the benchmark bar (the timed steps' own protocol: warm-up, then the timed steps) and on the
     12x12x50 bar released from a 90 degree twist (seed 11; 1 warm-up step, 30 timed).  other =
     (params, plain result) of the other arithmetic (mixed / FP64 throughout): the same network."""
     out = {}
-    model, model_bar, info = train_config4(P, params, dims)
+    model, model_bar, info = train_pooled_network(P, params, dims)
     out["training"] = info
     lw = timed_steps(P, rest, tets, mask, x0, params, args.warmup, args.steps, model)
     out["bench_bar"] = {"plain_steps_per_s": plain_sps, "plain_iterations_per_step": plain_its,

@@ -230,7 +230,7 @@ def run_oracle(i, params, steps, precision):
 
 @pytest.mark.parametrize("precision", [P.FP64, P.FP32])
 def test_batch256_instances_match_oracle(batch256, precision):
-    """Config 3 (96 descent iterations per step, dt = 1/30, gravity -z) in one batched solver, 2
+    """The 256-bar batch (96 descent iterations per step, dt = 1/30, gravity -z) in one batched solver, 2
     steps; the smallest, largest, softest and stiffest instance against the oracle stepping that
     instance alone: FP64 within 1e-7 of the bar's length with identical pass, iteration and
     rollback counts; FP32 within 1e-4 of the bar's length."""

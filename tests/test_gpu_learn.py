@@ -304,10 +304,10 @@ def test_trained_warm_start_gain():
     sm.close()
 
 
-def test_config1_warm_start_protocol():
-    """Configuration 4's protocol evaluated on configuration 1: the 12x12x50 bar released from a
-    90 degree twist (1 % jitter, seed 11), network trained on the plain solver's trajectories from
-    other seeds (2 and 3, impulses).  On this configuration the network does not beat the plain
+def test_bar12x12x50_warm_start_protocol():
+    """The benchmark's warm-start protocol on the 12x12x50 bar released from a 90 degree twist
+    (1 % jitter, seed 11), network trained on the plain solver's trajectories from other seeds (2
+    and 3, impulses).  On this bar the network does not beat the plain
     guess: its predicted corrections carry short-wavelength errors that the gradient test sees,
     and the guess is rejected in most steps.  What the test holds: every step converges to
     rel-grad 1e-4, the warm start costs at most 2 % of the plain iterations, and the trajectory
@@ -334,7 +334,7 @@ def test_config1_warm_start_protocol():
     ip = np.mean([q.iterations for q in stp])
     il = np.mean([q.iterations for q in stl])
     used = np.mean([q.used_learned for q in stl])
-    say("config 1: iterations/step plain %.1f, warm-started %.1f (%.2f%% saved), guess used in %.0f%% of steps"
+    say("12x12x50 bar: iterations/step plain %.1f, warm-started %.1f (%.2f%% saved), guess used in %.0f%% of steps"
         % (ip, il, 100 * (ip - il) / ip, 100 * used))
     assert all(q.converged and q.g_norm <= 1e-4 * q.g0_norm for q in stl)
     assert il <= 1.02 * ip
