@@ -485,6 +485,12 @@ __device__ __forceinline__ void tile_body(const DevParams &p, const Ctrl *ctrl, 
   C *rec = (C *)(sx + 4 * td.lcap);
   float4 *srest = (float4 *)(smem + tile_rest_offset<R>(td));
   const int nthr = blockDim.x;
+  // neo-Hookean: tet tid's local ids and slots, loaded before the staging so that they arrive with
+  // it (FP64 155 -> 148 us per launch at 6.3 M tets).  The stretch material's kernel would lose a
+  // wave per SIMD to the 4 registers it holds across the staging (FP32 137 -> 147 us), so it loads
+  // them after the barrier.
+  constexpr bool PRE = MAT == 0;
+  const uint4 lp0 = PRE && tid < ti.tet_cnt ? td.tet_lp[(size_t)ti.tet_start + tid] : make_uint4(0, 0, 0, 0);
   for (int i = tid; i < L; i += nthr) {
     const int g = i < ti.own_cnt ? ti.own_start + i : td.halo[ti.halo_off + i - ti.own_cnt];
     sx[4 * i] = x[g];
@@ -497,8 +503,7 @@ __device__ __forceinline__ void tile_body(const DevParams &p, const Ctrl *ctrl, 
   double e_el = 0.0;
   for (int phase = 0; phase <= hdnow; ++phase) {
     for (int s = tid; s < ti.tet_cnt; s += nthr) {
-      const size_t t = (size_t)ti.tet_start + s;
-      const uint4 lp = td.tet_lp[t];
+      const uint4 lp = PRE && s == tid ? lp0 : td.tet_lp[(size_t)ti.tet_start + s];
       const int lid[4] = {(int)(lp.x & 0xffff), (int)(lp.x >> 16), (int)(lp.y & 0xffff), (int)(lp.y >> 16)};
       float r[4][3];
 #pragma unroll
