@@ -643,6 +643,29 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
   }
 }
 
+// acc[k] += v[K i + k] for i = first, first + 1024, ... below end, in that order: the loads of 8
+// consecutive terms are issued before their additions (indices past the end are clamped to
+// `first`, whose value is then not added), so the thread waits for memory once per 8 terms
+// instead of once per term, and the sums are those of the plain loop.
+template <int K>
+__device__ __forceinline__ void strided_sum(const double *__restrict__ v, int first, int end, double *acc) {
+  constexpr int U = 8;
+  for (int base = first; base < end; base += U * 1024) {
+    double t[U][K];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * 1024 < end ? base + u * 1024 : first;
+#pragma unroll
+      for (int k = 0; k < K; ++k) t[u][k] = v[(size_t)K * i + k];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (base + u * 1024 < end)
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] += t[u][k];
+  }
+}
+
 // Control step, one workgroup per instance: FP64 sums in a fixed order of the instance's tile
 // energies (check passes) and node-block partials (energy, |g|^2), then the loop control of the
 // reference.
@@ -653,21 +676,23 @@ __global__ void __launch_bounds__(1024) k_control(DevParams p, Ctrl *__restrict_
   if (ctrl->done) return;
   const int tid = threadIdx.x;
   double a = 0.0, c = 0.0;
-  if (ctrl->check_now)
-    for (int i = td.inst_tile_off[inst] + tid; i < td.inst_tile_off[inst + 1]; i += 1024) a += td.tpart[i];
-  for (int i = td.inst_blk_off[inst] + tid; i < td.inst_blk_off[inst + 1]; i += 1024) {
-    a += td.npart[2 * i];
-    c += td.npart[2 * i + 1];
+  if (ctrl->check_now) strided_sum<1>(td.tpart, td.inst_tile_off[inst] + tid, td.inst_tile_off[inst + 1], &a);
+  {
+    double ac[2] = {a, c};
+    strided_sum<2>(td.npart, td.inst_blk_off[inst] + tid, td.inst_blk_off[inst + 1], ac);
+    a = ac[0];
+    c = ac[1];
   }
   const bool mix = p.f32math != 0;  // mixed mode: also |f|^2 of this pass, the FP32 force scale
   double f = 0.0;
-  if (mix)
-    for (int i = td.inst_blk_off[inst] + tid; i < td.inst_blk_off[inst + 1]; i += 1024) f += td.fpart[i];
+  if (mix) strided_sum<1>(td.fpart, td.inst_blk_off[inst] + tid, td.inst_blk_off[inst + 1], &f);
   la[tid] = a;
   lc[tid] = c;
   lf[tid] = f;
   __syncthreads();
-  for (int st = 512; st > 0; st >>= 1) {
+  // pairwise tree: levels 512 .. 64 through LDS, levels 32 .. 1 inside wavefront 0 by shuffles,
+  // which pair lane t with lane t + st exactly as the LDS levels pair slots
+  for (int st = 512; st >= 64; st >>= 1) {
     if (tid < st) {
       la[tid] += la[tid + st];
       lc[tid] += lc[tid + st];
@@ -675,12 +700,20 @@ __global__ void __launch_bounds__(1024) k_control(DevParams p, Ctrl *__restrict_
     }
     __syncthreads();
   }
-  const double E = la[0], g2 = lc[0], f2 = lf[0];
-  if (tid == 0) {
-    Ctrl C = *ctrl;
-    if (p.f32math) C.fnorm = sqrt(f2);
-    control_apply(p, C, E, g2);
-    *ctrl = C;
+  if (tid < 64) {
+    double E = la[tid], g2 = lc[tid], f2 = lf[tid];
+#pragma unroll
+    for (int st = 32; st > 0; st >>= 1) {
+      E += __shfl_down(E, st, 64);
+      g2 += __shfl_down(g2, st, 64);
+      if (mix) f2 += __shfl_down(f2, st, 64);
+    }
+    if (tid == 0) {
+      Ctrl C = *ctrl;
+      if (p.f32math) C.fnorm = sqrt(f2);
+      control_apply(p, C, E, g2);
+      *ctrl = C;
+    }
   }
 }
 
