@@ -45,19 +45,45 @@ __device__ __forceinline__ double block_sum256(double v, double *lds) {
 }
 
 // Sum of component c of the per-block partials, by one block of 1024 threads in a fixed order.
-__device__ __forceinline__ double partial_sum(const double *part, int nb, int c, double *lds) {
-  double v = 0.0;
-  for (int i = threadIdx.x; i < nb; i += 1024) v += part[4 * (size_t)i + c];
+template <int K>
+__device__ __forceinline__ void partial_sums(const double *part, int nb, const int (&c)[K], double *lds, double (&out)[K]) {
+  // per thread: part[4 i + c] for i = tid, tid + 1024, ... in that order, the loads of 8 terms
+  // issued before their additions (indices past the end clamped to the first, not added)
+  double v[K];
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  for (int k = 0; k < K; ++k) v[k] = 0.0;
+  const int first = threadIdx.x;
+  for (int base = first; base < nb; base += 8 * 1024) {
+    double t[8][K];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = base + u * 1024 < nb ? base + u * 1024 : first;
+#pragma unroll
+      for (int k = 0; k < K; ++k) t[u][k] = part[4 * (size_t)i + c[k]];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (base + u * 1024 < nb)
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] += t[u][k];
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_down(v[k], o, 64);
   const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) lds[w] = v;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) lds[16 * k + w] = v[k];
   __syncthreads();
-  double t = 0.0;
-  if (threadIdx.x == 0)
-    for (int i = 0; i < 16; ++i) t += lds[i];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double t = 0.0;
+    if (threadIdx.x == 0)
+      for (int i = 0; i < 16; ++i) t += lds[16 * k + i];
+    out[k] = t;
+  }
   __syncthreads();
-  return t;
 }
 
 // y += (row n of A) v.  BLK = 3: 3x3 blocks (9 values per slot); BLK = 1: one value per slot,
@@ -426,10 +452,10 @@ template <typename R, int BLK> __global__ void __launch_bounds__(kBlock) k_ie_st
 }
 
 __global__ void __launch_bounds__(1024) k_ie_start_ctrl(IeDev d, int nb) {
-  __shared__ double lds[16];
-  const double rz = partial_sum(d.part, nb, 0, lds);
-  const double rr = partial_sum(d.part, nb, 1, lds);
-  const double bb = partial_sum(d.part, nb, 2, lds);
+  __shared__ double lds[48];
+  double sums[3];
+  partial_sums<3>(d.part, nb, {0, 1, 2}, lds, sums);
+  const double rz = sums[0], rr = sums[1], bb = sums[2];
   if (threadIdx.x == 0) {
     IeCtrl c = {};
     c.rz = rz;
@@ -473,7 +499,9 @@ template <typename R, int BLK> __global__ void __launch_bounds__(kBlock) k_ie_sp
 __global__ void __launch_bounds__(1024) k_ie_alpha(IeDev d, int nb) {
   __shared__ double lds[16];
   if (d.ctrl->done) return;
-  const double pap = partial_sum(d.part, nb, 3, lds);
+  double sums[1];
+  partial_sums<1>(d.part, nb, {3}, lds, sums);
+  const double pap = sums[0];
   if (threadIdx.x == 0) {
     IeCtrl *c = d.ctrl;
     if (!(pap > 0.0)) {  // not positive definite (or a non-finite value): stop without a step
@@ -521,10 +549,11 @@ __global__ void __launch_bounds__(kBlock) k_ie_update(IeDev d) {
 }
 
 __global__ void __launch_bounds__(1024) k_ie_beta(IeDev d, int nb) {
-  __shared__ double lds[16];
+  __shared__ double lds[32];
   if (d.ctrl->done) return;
-  const double rz = partial_sum(d.part, nb, 0, lds);
-  const double rr = partial_sum(d.part, nb, 1, lds);
+  double sums[2];
+  partial_sums<2>(d.part, nb, {0, 1}, lds, sums);
+  const double rz = sums[0], rr = sums[1];
   if (threadIdx.x == 0) {
     IeCtrl *c = d.ctrl;
     c->k += 1;
