@@ -566,22 +566,39 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
   const int N = p.nn;
   double en = 0.0, g2 = 0.0, f2 = 0.0;
   if (n < td.inst_node_off[inst + 1]) {
-    R f[3];
-    float hv[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) f[d] = fown[(size_t)d * N + n];
+    // every load of the node that depends on no other is issued first (the flags are uniform), so
+    // the thread waits for memory about twice (these loads, then its halo partials), not once per
+    // array
     const int b0 = td.bnd_off[n], b1 = td.bnd_off[n + 1];
+    const int fixed = mask[n];
+    const float m = mass[n];
+    R f[3], xc[3], xtv[3], xlv[3];
+    float hv[3], ev[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const size_t i = (size_t)d * N + n;
+      f[d] = fown[i];
+      xc[d] = x[i];
+      xtv[d] = xt[i];
+      xlv[d] = (R)0;
+      ev[d] = 0.0f;
+      hv[d] = 0.0f;
+    }
+    if (!GRAD_ONLY && cheb)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) xlv[d] = xl[(size_t)d * N + n];
+    if (p.has_ext)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) ev[d] = ext[(size_t)d * N + n];
+    if (!GRAD_ONLY)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) hv[d] = hdnow ? fownh[(size_t)d * N + n] : hd[(size_t)d * N + n];
     for (int e = b0; e < b1; ++e) {
       const R *hp = hpart + 3 * (size_t)e;
 #pragma unroll
       for (int d = 0; d < 3; ++d) f[d] += hp[d];
     }
-    if (GRAD_ONLY) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) hv[d] = 0.0f;
-    } else if (hdnow) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) hv[d] = fownh[(size_t)d * N + n];
+    if (!GRAD_ONLY && hdnow) {
       for (int e = b0; e < b1; ++e) {
         const float *hp = hparth + 3 * (size_t)e;
 #pragma unroll
@@ -589,37 +606,28 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
       }
 #pragma unroll
       for (int d = 0; d < 3; ++d) hd[(size_t)d * N + n] = hv[d];
-    } else {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) hv[d] = hd[(size_t)d * N + n];
     }
-    if (mask[n]) {
+    if (fixed) {
       if (!GRAD_ONLY)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) xn[(size_t)d * N + n] = x[(size_t)d * N + n];
+        for (int d = 0; d < 3; ++d) xn[(size_t)d * N + n] = xc[d];
     } else {
-      const float m = mass[n];
       const R hstep = (R)p.h;
       const R mh2 = (R)m / (hstep * hstep);
       R e = (R)0;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        const size_t i = (size_t)d * N + n;
-        const R xc = x[i];
-        const R dd = xc - xt[i];
+        const R dd = xc[d] - xtv[d];
         R fe = (R)(p.grav[d] * m);  // = the setup's gravity force (mass before the fixed override)
-        if (p.has_ext) fe += (R)ext[i];
+        if (p.has_ext) fe += (R)ev[d];
         const R gr = dd * mh2 - f[d] - (R)p.ext_scale * fe;
-        e += (R)0.5 * dd * dd * mh2 - fe * xc;
+        e += (R)0.5 * dd * dd * mh2 - fe * xc[d];
         g2 += (double)gr * (double)gr;
         f2 += (double)f[d] * (double)f[d];
         if (!GRAD_ONLY) {
-          R nx = xc - alpha * gr / ((R)hv[d] + mh2);
-          if (cheb) {
-            const R l = xl[i];
-            nx = om * (nx - l) + l;
-          }
-          xn[i] = nx;
+          R nx = xc[d] - alpha * gr / ((R)hv[d] + mh2);
+          if (cheb) nx = om * (nx - xlv[d]) + xlv[d];
+          xn[(size_t)d * N + n] = nx;
         }
       }
       if (!GRAD_ONLY) en += (double)e;
