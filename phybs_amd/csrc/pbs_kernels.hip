@@ -422,13 +422,28 @@ __device__ __forceinline__ void tile_gather(int phase, const TileInfo &ti, const
       d0 = 3 * (size_t)td.halo_slot[ti.halo_off + (i - ti.own_cnt)];
       stride = 1;
     }
+    // the run is summed in order; FP32 contributions are read 4 at a time ahead of their additions
+    // (slots past the run clamped to its first slot and not added: FP32 tile pass 85.6 -> 82.4 us
+    // at 6.3 M tets), FP64 ones one at a time (4 ahead: 148 -> 152 us)
+    constexpr int U = sizeof(Q) == 4 ? 4 : 1;
     if (phase == 0) {
       R a0 = (R)0, a1 = (R)0, a2 = (R)0;
-      for (int j = o0; j < o1; ++j) {
-        const Q *r = rec + W * j;
-        a0 += (R)r[0];
-        a1 += (R)r[1];
-        a2 += (R)r[2];
+      for (int j = o0; j < o1; j += U) {
+        Q t[U][3];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const Q *r = rec + W * (j + u < o1 ? j + u : o0);
+          t[u][0] = r[0];
+          t[u][1] = r[1];
+          t[u][2] = r[2];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (j + u < o1) {
+            a0 += (R)t[u][0];
+            a1 += (R)t[u][1];
+            a2 += (R)t[u][2];
+          }
       }
       R *o = i < ti.own_cnt ? fown : hpart;
       o[d0] = a0;
@@ -436,11 +451,22 @@ __device__ __forceinline__ void tile_gather(int phase, const TileInfo &ti, const
       o[2 * stride + d0] = a2;
     } else {
       float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-      for (int j = o0; j < o1; ++j) {
-        const Q *r = rec + W * j;
-        a0 += (float)r[0];
-        a1 += (float)r[1];
-        a2 += (float)r[2];
+      for (int j = o0; j < o1; j += U) {
+        Q t[U][3];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const Q *r = rec + W * (j + u < o1 ? j + u : o0);
+          t[u][0] = r[0];
+          t[u][1] = r[1];
+          t[u][2] = r[2];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (j + u < o1) {
+            a0 += (float)t[u][0];
+            a1 += (float)t[u][1];
+            a2 += (float)t[u][2];
+          }
       }
       float *o = i < ti.own_cnt ? fownh : hparth;
       o[d0] = a0;
