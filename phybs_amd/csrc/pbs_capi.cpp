@@ -91,7 +91,8 @@ static void make_devparams(pbs_solver *s) {
 }
 
 // ---- kernel timing ----
-static int timed_begin(pbs_solver *s, int kid) {
+// A pair of events for one timed launch (kernel id kid, descent-pass ordinal s->pass_ordinal).
+static int timed_slot(pbs_solver *s, int kid) {
   if (!s->prof_period) return -1;
   if (s->pool_used + 2 > (int)s->pool.size()) {
     for (int i = 0; i < 64; ++i) {
@@ -102,8 +103,13 @@ static int timed_begin(pbs_solver *s, int kid) {
   }
   int i = s->pool_used;
   s->pool_used += 2;
-  (void)hipEventRecord(s->pool[i], s->st);
   s->pending.push_back({kid, i, s->pass_ordinal});
+  return i;
+}
+// Times the stream operations between timed_begin and timed_end (several launches).
+static int timed_begin(pbs_solver *s, int kid) {
+  const int i = timed_slot(s, kid);
+  if (i >= 0) (void)hipEventRecord(s->pool[i], s->st);
   return i;
 }
 static void timed_end(pbs_solver *s, int i) {
@@ -320,8 +326,6 @@ static void free_all(pbs_solver *s) {
   s->learn = nullptr;
   implicit_release(s->implicit);
   s->implicit = nullptr;
-  for (auto e : s->gev) (void)hipEventDestroy(e);
-  s->gev.clear();
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->st) (void)hipStreamDestroy(s->st);
@@ -853,43 +857,33 @@ extern "C" int pbs_state_load(pbs_solver *s, const char *path) {
 }
 
 // ---- the step ----
+// One descent pass; timed: each kernel's launch records its own start and end (the dispatch's
+// timestamps, as a kernel trace reports them).
 template <typename R> static void launch_pass(pbs_solver *s, bool timed) {
-  int i = timed ? timed_begin(s, s->dp.f32math ? K_TILE_MIXED : K_TILE) : -1;
-  Launch<R>::tile_pass(mesh<R>(s), s->td, s->dp, s->st);
-  timed_end(s, i);
-  i = timed ? timed_begin(s, K_NODE) : -1;
-  Launch<R>::node_pass(mesh<R>(s), s->td, s->dp, s->st);
-  timed_end(s, i);
-  i = timed ? timed_begin(s, K_CONTROL) : -1;
-  Launch<R>::control(mesh<R>(s), s->td, s->dp, s->st);
-  timed_end(s, i);
+  auto ev = [&](int i, int k) -> hipEvent_t { return i >= 0 ? s->pool[i + k] : nullptr; };
+  int i = timed ? timed_slot(s, s->dp.f32math ? K_TILE_MIXED : K_TILE) : -1;
+  Launch<R>::tile_pass(mesh<R>(s), s->td, s->dp, s->st, ev(i, 0), ev(i, 1));
+  i = timed ? timed_slot(s, K_NODE) : -1;
+  Launch<R>::node_pass(mesh<R>(s), s->td, s->dp, s->st, ev(i, 0), ev(i, 1));
+  i = timed ? timed_slot(s, K_CONTROL) : -1;
+  Launch<R>::control(mesh<R>(s), s->td, s->dp, s->st, ev(i, 0), ev(i, 1));
 }
 
-// Captures G descent passes into a HIP graph; the timed variant brackets each kernel with
-// external event-record nodes so that launch times are measured inside the replays.
-// Cached graphs: [timed + 2 * f32math] (plain / timed replays, FP64 or mixed-mode FP32 arithmetic).
-template <typename R> static int graph_build(pbs_solver *s, int G, int timed) {
-  auto &gs = s->graph[timed + 2 * (s->dp.f32math ? 1 : 0)];
+// Captures G descent passes into a HIP graph, cached per arithmetic (FP64 or mixed-mode FP32).
+// Timed replays are launched eagerly (launch_pass with timing), not from a graph.
+template <typename R> static int graph_build(pbs_solver *s, int G) {
+  auto &gs = s->graph[s->dp.f32math ? 1 : 0];
   if (gs.exec) (void)hipGraphExecDestroy(gs.exec);
   gs.exec = nullptr;
   gs.passes = 0;
-  while (timed && s->gev.size() < 6 * (size_t)G) {
-    hipEvent_t e;
-    HIPCHK(hipEventCreate(&e));
-    s->gev.push_back(e);
-  }
   HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
   hipError_t err = hipSuccess;
-  for (int j = 0; j < G && err == hipSuccess; ++j)
-    for (int k = 0; k < 3 && err == hipSuccess; ++k) {
-      if (timed) err = hipEventRecordWithFlags(s->gev[6 * j + 2 * k], s->st, hipEventRecordExternal);
-      if (err != hipSuccess) break;
-      if (k == 0) Launch<R>::tile_pass(mesh<R>(s), s->td, s->dp, s->st);
-      else if (k == 1) Launch<R>::node_pass(mesh<R>(s), s->td, s->dp, s->st);
-      else Launch<R>::control(mesh<R>(s), s->td, s->dp, s->st);
-      err = hipGetLastError();
-      if (timed && err == hipSuccess) err = hipEventRecordWithFlags(s->gev[6 * j + 2 * k + 1], s->st, hipEventRecordExternal);
-    }
+  for (int j = 0; j < G && err == hipSuccess; ++j) {
+    Launch<R>::tile_pass(mesh<R>(s), s->td, s->dp, s->st);
+    Launch<R>::node_pass(mesh<R>(s), s->td, s->dp, s->st);
+    Launch<R>::control(mesh<R>(s), s->td, s->dp, s->st);
+    err = hipGetLastError();
+  }
   hipGraph_t g = nullptr;
   hipError_t e2 = hipStreamEndCapture(s->st, &g);
   if (err == hipSuccess) err = e2;
@@ -904,23 +898,10 @@ template <typename R> static int graph_build(pbs_solver *s, int G, int timed) {
   return PBS_OK;
 }
 
-template <typename R> static int graph_ready(pbs_solver *s, int G, int timed) {
-  const auto &gs = s->graph[timed + 2 * (s->dp.f32math ? 1 : 0)];
+template <typename R> static int graph_ready(pbs_solver *s, int G) {
+  const auto &gs = s->graph[s->dp.f32math ? 1 : 0];
   if (gs.exec && gs.passes == G && std::memcmp(&gs.dp, &s->dp, sizeof(DevParams)) == 0) return PBS_OK;
-  return graph_build<R>(s, G, timed);
-}
-
-// Adds the launch times of the first `executed` passes of the last timed replay.
-static void graph_collect(pbs_solver *s, long executed) {
-  const int kid[3] = {s->dp.f32math ? K_TILE_MIXED : K_TILE, K_NODE, K_CONTROL};
-  for (long j = 0; j < executed && j < s->graph[1 + 2 * (s->dp.f32math ? 1 : 0)].passes; ++j)
-    for (int k = 0; k < 3; ++k) {
-      float ms = 0.f;
-      if (hipEventElapsedTime(&ms, s->gev[6 * j + 2 * k], s->gev[6 * j + 2 * k + 1]) == hipSuccess) {
-        s->kt_ms[kid[k]] += ms;
-        s->kt_n[kid[k]] += 1;
-      }
-    }
+  return graph_build<R>(s, G);
 }
 
 template <typename R> static int step_impl(pbs_solver *s, const float *ext, float *out, pbs_step_stats *stats) {
@@ -1005,21 +986,21 @@ template <typename R> static int step_impl(pbs_solver *s, const float *ext, floa
         if (launched > 0 && ((known && worst <= s->p.mixed_switch) || noisy)) fp64_phase = true;
       }
       s->dp.f32math = fp64_phase ? 0 : 1;
-      int rc = graph_ready<R>(s, G, 0);
-      if (!rc && period) rc = graph_ready<R>(s, G, 1);
+      int rc = graph_ready<R>(s, G);
       if (rc) return rc;
-      const int gi = 2 * s->dp.f32math;
+      const int gi = s->dp.f32math ? 1 : 0;
       const long reps = (n + G - 1) / G;
       for (long r = 0; r < reps; ++r) {
         const bool timed = period && (s->prof_tick++ % period) == 0;
-        if (timed) {  // the events are reused by the next timed replay: collect after this one
-          int rc = read_ctrl();
+        if (timed) {  // the same G passes launched eagerly, each kernel timing itself
+          for (long j = 0; j < G; ++j) {
+            s->pass_ordinal = launched + j;
+            launch_pass<R>(s, true);
+          }
+          HIPCHK(hipGetLastError());
+          int rc = read_ctrl();  // synchronises: the events are complete and can be reused
           if (rc) return rc;
-          const int before = max_passes();
-          HIPCHK(hipGraphLaunch(s->graph[gi + 1].exec, s->st));
-          rc = read_ctrl();
-          if (rc) return rc;
-          graph_collect(s, max_passes() - before);
+          timed_collect(s, max_passes());
         } else {
           HIPCHK(hipGraphLaunch(s->graph[gi].exec, s->st));
         }

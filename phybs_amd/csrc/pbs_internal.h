@@ -109,10 +109,15 @@ template <typename R> struct Launch {
   static void tet(const DevMesh<R> &m, const TileDev &td, const DevParams &p, int bs, int force_hd, int write_e,
                   hipStream_t st);
   static size_t tile_lds(const TileDev &td);
-  static void tile_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
-  static void node_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
+  // The kernels of a descent pass.  With events e0, e1 the launch records its own start and end
+  // in them (hipExtLaunchKernelGGL: the dispatch's timestamps, no extra stream operations).
+  static void tile_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st,
+                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+  static void node_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st,
+                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
   static void grad_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
-  static void control(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
+  static void control(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st,
+                      hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
   static void step_init(const DevMesh<R> &m, const TileDev &td, const DevParams &p, int bs, hipStream_t st);
   static void step_final(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st);
   static void standalone_ctrl(const DevMesh<R> &m, int cur, hipStream_t st, int hd = 0, int chk = 0);

@@ -15,6 +15,8 @@
 #include <algorithm>
 #include <type_traits>
 
+#include <hip/hip_ext.h>
+
 #include "pbs_device.h"
 #include "pbs_internal.h"
 
@@ -929,6 +931,14 @@ __global__ void k_fill(float *p, float v, long n) {
 // ---------------------------------------------------------------------------------------------
 static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
+// A launch that, given events, records its own start and end in them.
+template <typename F, typename... Args>
+static void launch_timed(F kernel, dim3 grid, dim3 block, size_t lds, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
+                         Args... args) {
+  if (e0) hipExtLaunchKernelGGL(kernel, grid, block, (std::uint32_t)lds, st, e0, e1, 0u, args...);
+  else hipLaunchKernelGGL(kernel, grid, block, lds, st, args...);
+}
+
 template <typename R>
 void Launch<R>::setup_tets(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st) {
   if (p.nt > 0) hipLaunchKernelGGL(k_setup_tets, dim3(nblk(p.nt, bs)), dim3(bs), 0, st, p, m.rest, m.tets, m.dmi, m.vol, m.flag);
@@ -952,12 +962,13 @@ size_t Launch<R>::tile_lds(const TileDev &td) {
   return tile_rest_offset<R>(td) + 16 * (size_t)td.lcap;
 }
 template <typename R>
-void Launch<R>::tile_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
+void Launch<R>::tile_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st, hipEvent_t e0,
+                          hipEvent_t e1) {
   const size_t lds = tile_lds(td);
   const bool f32m = sizeof(R) == 8 && p.f32math;
-#define PBS_TILE_LAUNCH(MAT, F32M)                                                                                \
-  hipLaunchKernelGGL((k_tile<R, MAT, F32M>), dim3(td.ntiles), dim3(td.threads), lds, st, p, m.ctrl, m.ring, td, \
-                     m.rest, m.fown, m.hpart, m.fownh, m.hparth)
+#define PBS_TILE_LAUNCH(MAT, F32M)                                                                               \
+  launch_timed(k_tile<R, MAT, F32M>, dim3(td.ntiles), dim3(td.threads), lds, st, e0, e1, p, m.ctrl, m.ring, td, \
+               m.rest, m.fown, m.hpart, m.fownh, m.hparth)
   if (p.material == 0 && !f32m) PBS_TILE_LAUNCH(0, false);
   else if (p.material == 0) PBS_TILE_LAUNCH(0, true);
   else if (!f32m) PBS_TILE_LAUNCH(1, false);
@@ -965,9 +976,10 @@ void Launch<R>::tile_pass(const DevMesh<R> &m, const TileDev &td, const DevParam
 #undef PBS_TILE_LAUNCH
 }
 template <typename R>
-void Launch<R>::node_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
-  hipLaunchKernelGGL((k_node<R, false>), dim3(td.nblk), dim3(256), 0, st, p, m.ctrl, m.ring, td, m.fown, m.hpart,
-                     m.fownh, m.hparth, m.hd, m.mass, m.mask, m.ext, m.xt);
+void Launch<R>::node_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st, hipEvent_t e0,
+                          hipEvent_t e1) {
+  launch_timed(k_node<R, false>, dim3(td.nblk), dim3(256), 0, st, e0, e1, p, m.ctrl, m.ring, td, m.fown, m.hpart,
+               m.fownh, m.hparth, m.hd, m.mass, m.mask, m.ext, m.xt);
 }
 template <typename R>
 void Launch<R>::grad_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
@@ -975,8 +987,9 @@ void Launch<R>::grad_pass(const DevMesh<R> &m, const TileDev &td, const DevParam
                      m.fownh, m.hparth, m.hd, m.mass, m.mask, m.ext, m.xt);
 }
 template <typename R>
-void Launch<R>::control(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
-  hipLaunchKernelGGL(k_control, dim3(td.ninst), dim3(1024), 0, st, p, m.ctrl, td);
+void Launch<R>::control(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st, hipEvent_t e0,
+                        hipEvent_t e1) {
+  launch_timed(k_control, dim3(td.ninst), dim3(1024), 0, st, e0, e1, p, m.ctrl, td);
 }
 template <typename R>
 void Launch<R>::step_init(const DevMesh<R> &m, const TileDev &td, const DevParams &p, int bs, hipStream_t st) {

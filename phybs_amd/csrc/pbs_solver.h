@@ -66,14 +66,13 @@ struct pbs_solver {
   std::vector<Pending> pending;
   int pool_used = 0;
   long pass_ordinal = -1;  // ordinal given to the descent-pass launches being recorded
-  // HIP graphs of graph_passes descent passes, replayed by pbs_step: [0] plain, [1] with
-  // timing events around each kernel (6 per pass, gev)
+  // HIP graphs of graph_passes descent passes, replayed by pbs_step: [0] FP64 arithmetic,
+  // [1] mixed-mode FP32 arithmetic
   struct PassGraph {
     hipGraphExec_t exec = nullptr;
     DevParams dp{};  // parameters baked into the graph
     int passes = 0;
-  } graph[4];
-  std::vector<hipEvent_t> gev;
+  } graph[2];
   int prof_period = 0;  // 0: no timing; n: time every n-th graph replay (or pass without graphs)
   long prof_tick = 0;
   double kt_ms[K_COUNT] = {0};
