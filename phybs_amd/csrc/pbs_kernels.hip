@@ -288,6 +288,15 @@ __device__ void control_apply(const DevParams &p, Ctrl &C, double E, double g2) 
   }
 }
 
+// The fields of a control block that a descent pass's tile and node kernels read.
+struct PassCtl {
+  int done, cur, nxt, last, hd_now, check_now, cheb_now;
+  float alpha, omega_next;
+};
+__device__ __forceinline__ PassCtl pass_ctl(const Ctrl *c) {
+  return {c->done, c->cur, c->nxt, c->last, c->hd_now, c->check_now, c->cheb_now, c->alpha, c->omega_next};
+}
+
 // XCD-aware tile order: workgroups are dispatched round-robin over the 8 XCDs, so block i runs
 // on XCD i % 8.  Giving each XCD a contiguous range of tiles keeps neighbouring tiles (which
 // share halo nodes and halo tets) in the same 4 MB L2.
@@ -503,12 +512,10 @@ template <typename R> __host__ __device__ inline size_t tile_rest_offset(const T
 // One tile, all phases: stage the local nodes' positions and rest positions, evaluate the tets
 // (phase 0 forces and energies; phase 1, when hd, the stiffness diagonal), gather each phase.
 template <typename R, int MAT, bool F32M>
-__device__ __forceinline__ void tile_body(const DevParams &p, const Ctrl *ctrl, const Ring<R> &ring, const TileDev &td,
+__device__ __forceinline__ void tile_body(const DevParams &p, int chk, const R *__restrict__ x, const TileDev &td,
                                           int b, const TileInfo &ti, int hdnow, const float *__restrict__ rest,
                                           R *__restrict__ fown, R *__restrict__ hpart, float *__restrict__ fownh,
                                           float *__restrict__ hparth, unsigned char *smem, double *lds_red) {
-  const int chk = ctrl->check_now;
-  const R *__restrict__ x = ring.b[ctrl->cur];
   const int L = ti.own_cnt + ti.halo_cnt, N = p.nn, tid = threadIdx.x;
   typedef typename std::conditional<F32M, float, R>::type C;
   R *sx = (R *)smem;
@@ -521,6 +528,7 @@ __device__ __forceinline__ void tile_body(const DevParams &p, const Ctrl *ctrl, 
   // them after the barrier.
   constexpr bool PRE = MAT == 0;
   const uint4 lp0 = PRE && tid < ti.tet_cnt ? td.tet_lp[(size_t)ti.tet_start + tid] : make_uint4(0, 0, 0, 0);
+  const C mu = (C)td.inst_mu[ti.inst], lam = (C)td.inst_lam[ti.inst];
   for (int i = tid; i < L; i += nthr) {
     const int g = i < ti.own_cnt ? ti.own_start + i : td.halo[ti.halo_off + i - ti.own_cnt];
     sx[4 * i] = x[g];
@@ -529,7 +537,6 @@ __device__ __forceinline__ void tile_body(const DevParams &p, const Ctrl *ctrl, 
     srest[i] = make_float4(rest[g], rest[N + g], rest[2 * (size_t)N + g], 0.0f);
   }
   __syncthreads();
-  const C mu = (C)td.inst_mu[ti.inst], lam = (C)td.inst_lam[ti.inst];
   double e_el = 0.0;
   // the tile's tets and their gather for one phase
   auto run_phase = [&](int phase) {
@@ -581,10 +588,18 @@ k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double lds_red[16];
   const int b = xcd_tile(blockIdx.x, gridDim.x);
+  // One instance: its control block is ctrl[0], read together with the tile descriptor rather than
+  // after it, so the staging loads wait for one dependent memory access less.  Several: the
+  // tile's instance's block, read once the descriptor has arrived.
+  PassCtl c = pass_ctl(ctrl);
   const TileInfo ti = td.tiles[b];
-  ctrl += ti.inst;
-  if (ctrl->done) return;
-  tile_body<R, MAT, F32M>(p, ctrl, ring, td, b, ti, ctrl->hd_now, rest, fown, hpart, fownh, hparth, smem, lds_red);
+  // both reads in flight before either is waited for (otherwise the compiler folds the first into
+  // the second, which waits for the descriptor)
+  asm volatile("" ::"s"(c.done), "s"(c.cur), "s"(c.hd_now), "s"(c.check_now), "s"(ti.inst));
+  if (td.ninst != 1) c = pass_ctl(ctrl + ti.inst);
+  if (c.done) return;
+  tile_body<R, MAT, F32M>(p, c.check_now, ring.b[c.cur], td, b, ti, c.hd_now, rest, fown, hpart, fownh, hparth, smem,
+                          lds_red);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -600,18 +615,28 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
        float *__restrict__ hd, const float *__restrict__ mass, const int *__restrict__ mask,
        const float *__restrict__ ext, const R *__restrict__ xt) {
   __shared__ double lds_red[48];
-  const int inst = td.blk_inst[blockIdx.x];
-  ctrl += inst;
-  if (ctrl->done) return;
-  const int hdnow = ctrl->hd_now, cheb = ctrl->cheb_now;
-  const R alpha = (R)ctrl->alpha, om = (R)ctrl->omega_next;
-  const R *__restrict__ x = ring.b[ctrl->cur];
-  const R *__restrict__ xl = ring.b[ctrl->last];
-  R *__restrict__ xn = ring.b[ctrl->nxt];
-  const int n = td.blk_start[blockIdx.x] + (int)threadIdx.x;
+  // One instance: control block ctrl[0], node blocks of 256 from node 0, read without waiting for
+  // the block's instance.  Several: the block's instance first.
+  PassCtl c = pass_ctl(ctrl);
+  asm volatile("" ::"s"(c.done), "s"(c.cur), "s"(c.nxt), "s"(c.last), "s"(c.hd_now), "s"(c.cheb_now), "s"(c.alpha),
+               "s"(c.omega_next));  // read now, not folded into the several-instance read below
+  int n0 = 256 * (int)blockIdx.x, nend = p.nn;
+  if (td.ninst != 1) {
+    const int inst = td.blk_inst[blockIdx.x];
+    c = pass_ctl(ctrl + inst);
+    n0 = td.blk_start[blockIdx.x];
+    nend = td.inst_node_off[inst + 1];
+  }
+  if (c.done) return;
+  const int hdnow = c.hd_now, cheb = c.cheb_now;
+  const R alpha = (R)c.alpha, om = (R)c.omega_next;
+  const R *__restrict__ x = ring.b[c.cur];
+  const R *__restrict__ xl = ring.b[c.last];
+  R *__restrict__ xn = ring.b[c.nxt];
+  const int n = n0 + (int)threadIdx.x;
   const int N = p.nn;
   double en = 0.0, g2 = 0.0, f2 = 0.0;
-  if (n < td.inst_node_off[inst + 1]) {
+  if (n < nend) {
     // every load of the node that depends on no other is issued first (the flags are uniform), so
     // the thread waits for memory about twice (these loads, then its halo partials), not once per
     // array
