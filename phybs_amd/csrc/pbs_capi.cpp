@@ -407,7 +407,11 @@ static int create_impl(int ninst, const int *node_off, const int *tet_off, const
   }
   // Tile size (tets per tile): 256, halved while the mesh would give fewer than 1024 tiles, and
   // while the tile pass could need more than 40 KiB of LDS (four workgroups per 160 KiB CU); a
-  // plan with tile_tets = t has at most t tets and max(64, t / 2) local nodes per tile.
+  // plan with tile_tets = t has at most t tets and max(64, t / 2) local nodes per tile.  FP64
+  // neo-Hookean arithmetic (not the mixed mode) takes 224 instead of 256: 27.0 KB of LDS per
+  // workgroup instead of 30.8 KB lets 6 workgroups share a CU, as many as the kernel's 6 waves
+  // per SIMD allow (FP64 tile pass 135.8 -> 131.3 us at 6.3 M tets; 224-tet tiles alone, at 5
+  // workgroups per CU, take 137.7 us).
   const bool f64 = p->precision == PBS_FP64;
   auto lds_need = [&](int lcap, int tcap) {
     TileDev t{};
@@ -419,6 +423,7 @@ static int create_impl(int ninst, const int *node_off, const int *tet_off, const
   if (tt == 0) {
     tt = 256;
     while (tt > 64 && (nt / tt < 1024 || lds_need(std::max(64, tt / 2), tt) > std::min<size_t>(s->lds_budget, 40960))) tt /= 2;
+    if (tt == 256 && f64 && p->material == 0 && !(p->mixed_switch > 0)) tt = 224;
   }
   std::string msg;
   const bool planned = inst_tiles ? build_batch_plan(ninst, node_off, tet_off, rest, tets, inst_tiles, s->plan, msg)

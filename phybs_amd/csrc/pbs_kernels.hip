@@ -581,8 +581,10 @@ __device__ __forceinline__ void tile_body(const DevParams &p, int chk, const R *
 }
 
 // F32M (FP64 state only): the per-tet arithmetic in FP32 after F (the mixed mode's early passes).
+// The FP64 neo-Hookean kernel is held to 6 waves per SIMD (at most 80 VGPRs: it needs 78 without
+// spilling), which pays with tiles small enough for 6 workgroups' LDS (224 tets, see pbs_create).
 template <typename R, int MAT, bool F32M>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, (sizeof(R) == 8 && MAT == 0 && !F32M) ? 6 : 1)
 k_tile(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, const float *__restrict__ rest,
        R *__restrict__ fown, R *__restrict__ hpart, float *__restrict__ fownh, float *__restrict__ hparth) {
   extern __shared__ __align__(16) unsigned char smem[];
