@@ -27,7 +27,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (each scaled by the
 # same counter's reading for a plain elementwise kernel of known size) on the 64x64x256 bar:
 # (kernel, bytes per position component) -> bytes.  These are not measured by this script (a
-# counter run serialises every dispatch); the bench line names the profile they come from.
+# counter run serialises every dispatch); the bench line names the profile they come from.  All
+# four were measured with 256-tet tiles (FP64 neo-Hookean solvers now default to 224).
 TRAFFIC_PROFILE = "profiles/k_tile_k_node_bar64x64x256_relgrad_f64.md"
 MEASURED_TRAFFIC = {("k_tile", 8): 237.6e6, ("k_tile", 4): 188.7e6, ("k_node", 8): 202.4e6, ("k_node", 4): 118.5e6}
 
@@ -505,6 +506,7 @@ def main():
     t1 = time.perf_counter()
     elapsed = t1 - t0
     kt = s.kernel_times()
+    tile_used = s.get_params().tile_tets
     iters = sum(st.iterations for st in stats)
     passes = sum(st.passes for st in stats)
     nt, nn = len(tets), len(rest)
@@ -554,7 +556,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": MEASURED_TRAFFIC.get((dom, rsz)) if args.workload == "bar64x64x256" else None,
-                     "traffic_source": (TRAFFIC_PROFILE + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; not this run)")
+                     "traffic_source": (TRAFFIC_PROFILE + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE with 256-tet tiles; "
+                                        "not this run, tiles of %d tets)" % tile_used)
                      if args.workload == "bar64x64x256" and (dom, rsz) in MEASURED_TRAFFIC else None,
                      "kernel": dom,
                      "algorithmic_bytes_per_launch": alg[dom]},
