@@ -562,7 +562,7 @@ __device__ __forceinline__ void tile_body(const DevParams &p, int chk, const R *
     // FP64 arithmetic: the two phases as separate code.  As one loop over the phase the compiler
     // shares their common arithmetic and keeps its constants in registers, and the neo-Hookean
     // kernel needs 114 VGPRs (4 waves per SIMD); as separate code, 94 (5 waves; 144 -> 139 us
-    // per launch at 6.3 M tets).
+    // per launch at 6.3 M tets), and 78 with the logarithm's constants in scalar registers.
     run_phase(0);
     if (hdnow) {
       __syncthreads();  // the Hessian phase reuses the contribution slots
