@@ -22,10 +22,8 @@ void implicit_params_changed(ImplicitState *S) {
 
 void implicit_release(ImplicitState *S) {
   if (!S) return;
-  for (CgGraphs *G : {&S->cg_ie, &S->cg_pd}) {
+  for (CgGraphs *G : {&S->cg_ie, &S->cg_pd})
     if (G->graph) (void)hipGraphExecDestroy(G->graph);
-    if (G->tgraph) (void)hipGraphExecDestroy(G->tgraph);
-  }
   for (auto e : S->ev) (void)hipEventDestroy(e);
   for (void *q : S->allocs) (void)hipFree(q);
   delete S;
@@ -138,56 +136,54 @@ int implicit_get(pbs_solver *s, ImplicitState **out) {
   return PBS_OK;
 }
 
-// One graph holds kGraphIters CG iterations on the matrix of `dev`; the timed variant brackets
-// each SpMV and each vector update with external event records.
-template <typename R> int cg_graph(pbs_solver *s, ImplicitState *S, const IeDev &dev, CgGraphs &G, int timed) {
-  const double key[2] = {dev.tol2, (double)dev.maxit};
-  if ((timed ? G.tgraph : G.graph) && std::memcmp(key, G.key, sizeof(key)) == 0) return PBS_OK;
-  if (G.graph) (void)hipGraphExecDestroy(G.graph);
-  if (G.tgraph) (void)hipGraphExecDestroy(G.tgraph);
-  G.graph = G.tgraph = nullptr;
-  std::memcpy(G.key, key, sizeof(key));
-  for (int t = 0; t < 2; ++t) {
-    if (t == 1 && !timed) break;
-    while (t == 1 && S->ev.size() < 4 * (size_t)kGraphIters + 2) {
-      hipEvent_t ev;
-      HIPCHK(hipEventCreate(&ev));
-      S->ev.push_back(ev);
-    }
-    HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
-    hipError_t err = hipSuccess;
-    for (int i = 0; i < kGraphIters && err == hipSuccess; ++i) {
-      if (t) err = hipEventRecordWithFlags(S->ev[4 * i], s->st, hipEventRecordExternal);
-      IeLaunch<R>::spmv(dev, s->st);
-      if (t && err == hipSuccess) err = hipEventRecordWithFlags(S->ev[4 * i + 1], s->st, hipEventRecordExternal);
-      IeLaunch<R>::alpha(dev, s->st);
-      if (t && err == hipSuccess) err = hipEventRecordWithFlags(S->ev[4 * i + 2], s->st, hipEventRecordExternal);
-      IeLaunch<R>::update(dev, s->st);
-      if (t && err == hipSuccess) err = hipEventRecordWithFlags(S->ev[4 * i + 3], s->st, hipEventRecordExternal);
-      IeLaunch<R>::beta(dev, s->st);
-      if (err == hipSuccess) err = hipGetLastError();
-    }
-    hipGraph_t gr = nullptr;
-    hipError_t e2 = hipStreamEndCapture(s->st, &gr);
-    if (err == hipSuccess) err = e2;
-    hipGraphExec_t ex = nullptr;
-    if (err == hipSuccess) err = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
-    if (gr) (void)hipGraphDestroy(gr);
-    if (err != hipSuccess) return set_error(PBS_ERR_HIP, std::string("CG graph capture: ") + hipGetErrorString(err));
-    (t ? G.tgraph : G.graph) = ex;
+// Timing events: 4 per CG iteration of a replay (SpMV and update dispatches), 2 around the
+// assembly or the local step.
+static int cg_events(ImplicitState *S) {
+  while (S->ev.size() < 4 * (size_t)kGraphIters + 2) {
+    hipEvent_t ev;
+    HIPCHK(hipEventCreate(&ev));
+    S->ev.push_back(ev);
   }
   return PBS_OK;
 }
 
+// One graph holds kGraphIters CG iterations on the matrix of `dev`.
+template <typename R> int cg_graph(pbs_solver *s, const IeDev &dev, CgGraphs &G) {
+  const double key[2] = {dev.tol2, (double)dev.maxit};
+  if (G.graph && std::memcmp(key, G.key, sizeof(key)) == 0) return PBS_OK;
+  if (G.graph) (void)hipGraphExecDestroy(G.graph);
+  G.graph = nullptr;
+  std::memcpy(G.key, key, sizeof(key));
+  HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+  hipError_t err = hipSuccess;
+  for (int i = 0; i < kGraphIters && err == hipSuccess; ++i) {
+    IeLaunch<R>::spmv(dev, s->st);
+    IeLaunch<R>::alpha(dev, s->st);
+    IeLaunch<R>::update(dev, s->st);
+    IeLaunch<R>::beta(dev, s->st);
+    err = hipGetLastError();
+  }
+  hipGraph_t gr = nullptr;
+  hipError_t e2 = hipStreamEndCapture(s->st, &gr);
+  if (err == hipSuccess) err = e2;
+  hipGraphExec_t ex = nullptr;
+  if (err == hipSuccess) err = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+  if (gr) (void)hipGraphDestroy(gr);
+  if (err != hipSuccess) return set_error(PBS_ERR_HIP, std::string("CG graph capture: ") + hipGetErrorString(err));
+  G.graph = ex;
+  return PBS_OK;
+}
+
 // Solves dev's system from the start vector in dev.x (b, dinv already set): the start kernels,
-// then graph replays until the device-side test stops; with profiling on, the SpMV and update
-// launches of the replays are timed.  c receives the final control block.
+// then graph replays until the device-side test stops; with profiling on, the same iterations are
+// launched eagerly instead, the SpMV and update dispatches carrying their own start and end
+// events.  c receives the final control block.
 template <typename R>
 int cg_solve(pbs_solver *s, ImplicitState *S, IeDev &dev, CgGraphs &G, double tol, int maxit, IeCtrl &c) {
   dev.tol2 = tol * tol;
   dev.maxit = maxit;
   const bool timed = s->prof_period > 0;
-  int rc = cg_graph<R>(s, S, dev, G, timed ? 1 : 0);
+  int rc = timed ? cg_events(S) : cg_graph<R>(s, dev, G);
   if (rc) return rc;
   IeLaunch<R>::start(dev, s->st);
   HIPCHK(hipGetLastError());
@@ -198,7 +194,17 @@ int cg_solve(pbs_solver *s, ImplicitState *S, IeDev &dev, CgGraphs &G, double to
   while (!c.done) {
     if (replays++ >= max_replays) return set_error(PBS_ERR_STATE, "CG loop did not stop within its iteration cap");
     const int k_before = c.k;
-    HIPCHK(hipGraphLaunch(timed ? G.tgraph : G.graph, s->st));
+    if (timed) {
+      for (int i = 0; i < kGraphIters; ++i) {
+        IeLaunch<R>::spmv(dev, s->st, S->ev[4 * i], S->ev[4 * i + 1]);
+        IeLaunch<R>::alpha(dev, s->st);
+        IeLaunch<R>::update(dev, s->st, S->ev[4 * i + 2], S->ev[4 * i + 3]);
+        IeLaunch<R>::beta(dev, s->st);
+      }
+      HIPCHK(hipGetLastError());
+    } else {
+      HIPCHK(hipGraphLaunch(G.graph, s->st));
+    }
     HIPCHK(hipMemcpyAsync(&c, dev.ctrl, sizeof(IeCtrl), hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
     if (timed) {  // iterations that ran: k advanced by one per completed iteration (+1 if stopped in alpha)
@@ -266,9 +272,7 @@ int implicit_step_impl(pbs_solver *s, const pbs_implicit_params *ip, const float
   a.has_ext = ext ? 1 : 0;
   const bool timed = s->prof_period > 0;
   if (timed) {
-    d.tol2 = ip->cg_tolerance * ip->cg_tolerance;
-    d.maxit = ip->cg_max_iterations;
-    rc = cg_graph<R>(s, S, d, S->cg_ie, 1);  // creates the timing events
+    rc = cg_events(S);
     if (rc) return rc;
   }
   HIPCHK(hipEventRecord(s->ev0, s->st));
@@ -362,9 +366,7 @@ int pd_step_impl(pbs_solver *s, const pbs_pd_params *pp, const float *ext, float
   int conv = 1;
   const bool timed = s->prof_period > 0;
   if (timed) {
-    d.tol2 = pp->cg_tolerance * pp->cg_tolerance;
-    d.maxit = pp->cg_max_iterations;
-    rc = cg_graph<R>(s, S, d, S->cg_pd, 1);  // creates the timing events
+    rc = cg_events(S);
     if (rc) return rc;
   }
   hipEvent_t *ea = timed ? &S->ev[4 * kGraphIters] : nullptr;  // events around the local step

@@ -49,10 +49,11 @@ template <typename R> struct IeLaunch {
   // per-tet 12 x 12 stiffness of the solver tets ids[0 .. num) at positions x (planes), FP32 out
   static void tet_stiffness(const DevMesh<R> &m, const DevParams &p, const float *inst_mu, const float *inst_lam,
                             const R *x, int num, const int *ids, float *out, hipStream_t st);
-  // one CG iteration = spmv, alpha, update, beta (+ the search-direction update), each a kernel
-  static void spmv(const IeDev &d, hipStream_t st);
+  // one CG iteration = spmv, alpha, update, beta (+ the search-direction update), each a kernel;
+  // e0, e1 (optional): events recording the start and end of the kernel's own dispatch
+  static void spmv(const IeDev &d, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
   static void alpha(const IeDev &d, hipStream_t st);
-  static void update(const IeDev &d, hipStream_t st);
+  static void update(const IeDev &d, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
   static void beta(const IeDev &d, hipStream_t st);
   static void finish(const DevMesh<R> &m, const DevParams &p, float h, const IeDev &d, hipStream_t st);
   // projective dynamics: global matrix, momentum term + iterate start, local step, velocity update
@@ -65,10 +66,10 @@ template <typename R> struct IeLaunch {
   static void pd_finish(const DevMesh<R> &m, const DevParams &p, float damping, const IeDev &d, hipStream_t st);
 };
 
-// Captured CG iterations for one matrix (plain replay and the timed variant).
+// Captured CG iterations for one matrix.
 struct CgGraphs {
-  hipGraphExec_t graph = nullptr, tgraph = nullptr;
-  double key[2] = {0, 0};  // tol2, maxit the graphs were built with
+  hipGraphExec_t graph = nullptr;
+  double key[2] = {0, 0};  // tol2, maxit the graph was built with
 };
 
 struct ImplicitState {

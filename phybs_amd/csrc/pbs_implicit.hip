@@ -8,6 +8,8 @@
 // 3x12 stiffness strip of its corner in closed form and accumulates it into its own SELL-64
 // slots, so the assembly is deterministic and atomic-free; the solve is a Jacobi-preconditioned
 // conjugate gradient in FP64 whose scalars stay on the device.
+#include <hip/hip_ext.h>
+
 #include "pbs_device.h"
 #include "pbs_implicit.h"
 
@@ -799,15 +801,21 @@ template <typename R> void IeLaunch<R>::start(const IeDev &d, hipStream_t st) {
   hipLaunchKernelGGL(k_ie_start_ctrl, dim3(1), dim3(1024), 0, st, d, nb_of(d.n));
 }
 
-template <typename R> void IeLaunch<R>::spmv(const IeDev &d, hipStream_t st) {
-  if (d.scalar) hipLaunchKernelGGL((k_ie_spmv<R, 1>), dim3(nb_of(d.n)), dim3(kBlock), 0, st, d);
-  else hipLaunchKernelGGL((k_ie_spmv<R, 3>), dim3(nb_of(d.n)), dim3(kBlock), 0, st, d);
+// e0 set: the launch records the dispatch's start and end in e0, e1 (as a kernel trace times it)
+template <typename F>
+static void ie_launch(F kernel, dim3 grid, dim3 block, hipStream_t st, hipEvent_t e0, hipEvent_t e1, const IeDev &d) {
+  if (e0) hipExtLaunchKernelGGL(kernel, grid, block, 0u, st, e0, e1, 0u, d);
+  else hipLaunchKernelGGL(kernel, grid, block, 0, st, d);
+}
+template <typename R> void IeLaunch<R>::spmv(const IeDev &d, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+  if (d.scalar) ie_launch(k_ie_spmv<R, 1>, dim3(nb_of(d.n)), dim3(kBlock), st, e0, e1, d);
+  else ie_launch(k_ie_spmv<R, 3>, dim3(nb_of(d.n)), dim3(kBlock), st, e0, e1, d);
 }
 template <typename R> void IeLaunch<R>::alpha(const IeDev &d, hipStream_t st) {
   hipLaunchKernelGGL(k_ie_alpha, dim3(1), dim3(1024), 0, st, d, nb_of(d.n));
 }
-template <typename R> void IeLaunch<R>::update(const IeDev &d, hipStream_t st) {
-  hipLaunchKernelGGL(k_ie_update, dim3((3 * d.n + kUpd - 1) / kUpd), dim3(kBlock), 0, st, d);
+template <typename R> void IeLaunch<R>::update(const IeDev &d, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+  ie_launch(k_ie_update, dim3((3 * d.n + kUpd - 1) / kUpd), dim3(kBlock), st, e0, e1, d);
 }
 template <typename R> void IeLaunch<R>::beta(const IeDev &d, hipStream_t st) {
   hipLaunchKernelGGL(k_ie_beta, dim3(1), dim3(1024), 0, st, d, (3 * d.n + kUpd - 1) / kUpd);
