@@ -190,6 +190,7 @@ template <typename R> static int alloc_impl(pbs_solver *s) {
   rc |= dalloc(s, &m.perm, N);
   rc |= dalloc(s, &m.mask, N);
   rc |= dalloc(s, &m.mass, N);
+  rc |= dalloc(s, &m.mnode, N);
   rc |= dalloc(s, &m.gforce, 3 * N);
   rc |= dalloc(s, &m.ext, 3 * N);
   rc |= dalloc(s, &m.hd, 3 * N);

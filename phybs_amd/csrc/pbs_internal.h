@@ -83,6 +83,8 @@ template <typename R> struct DevMesh {
   int *perm;        // N: solver node -> caller node
   int *mask;        // N
   float *mass;      // N (fixed nodes: fixed_mass)
+  float *mnode;     // N: the computed mass of a free node, -1 for a fixed node (the node pass reads
+                    // this one array instead of mask and mass)
   float *gforce;    // 3 planes: mass (before the fixed override) * grav
   float *ext;       // 3 planes
   float *hd;        // 3 planes: Hessian diagonal

@@ -150,7 +150,7 @@ __global__ void k_setup_tets(DevParams p, const float *__restrict__ rest, const 
 // before constrained nodes get fixed_mass (CUDAInterface.cu:716-719).
 __global__ void k_setup_nodes(DevParams p, const int *__restrict__ inc_off, const int *__restrict__ inc,
                               const float *__restrict__ vol, const int *__restrict__ mask, float *__restrict__ mass,
-                              float *__restrict__ gforce) {
+                              float *__restrict__ mnode, float *__restrict__ gforce) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= p.nn) return;
   float m = 0.0f;
@@ -161,6 +161,7 @@ __global__ void k_setup_nodes(DevParams p, const int *__restrict__ inc_off, cons
 #pragma unroll
   for (int d = 0; d < 3; ++d) gforce[(size_t)d * p.nn + n] = p.grav[d] * m;
   mass[n] = mask[n] ? p.fixed_mass : m;
+  mnode[n] = mask[n] ? -1.0f : m;
 }
 #pragma clang fp contract(on)
 
@@ -614,8 +615,8 @@ template <typename R, bool GRAD_ONLY>
 __global__ void __launch_bounds__(256)
 k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, const R *__restrict__ fown,
        const R *__restrict__ hpart, const float *__restrict__ fownh, const float *__restrict__ hparth,
-       float *__restrict__ hd, const float *__restrict__ mass, const int *__restrict__ mask,
-       const float *__restrict__ ext, const R *__restrict__ xt) {
+       float *__restrict__ hd, const float *__restrict__ mnode, const float *__restrict__ ext,
+       const R *__restrict__ xt) {
   __shared__ double lds_red[48];
   // One instance: control block ctrl[0], node blocks of 256 from node 0, read without waiting for
   // the block's instance.  Several: the block's instance first.
@@ -643,8 +644,8 @@ k_node(DevParams p, const Ctrl *__restrict__ ctrl, Ring<R> ring, TileDev td, con
     // the thread waits for memory about twice (these loads, then its halo partials), not once per
     // array
     const int b0 = td.bnd_off[n], b1 = td.bnd_off[n + 1];
-    const int fixed = mask[n];
-    const float m = mass[n];
+    const float m = mnode[n];  // the node's mass, or -1: fixed
+    const bool fixed = m < 0.0f;
     R f[3], xc[3], xtv[3], xlv[3];
     float hv[3], ev[3];
 #pragma unroll
@@ -990,7 +991,8 @@ void Launch<R>::setup_tets(const DevMesh<R> &m, const DevParams &p, int bs, hipS
 }
 template <typename R>
 void Launch<R>::setup_nodes(const DevMesh<R> &m, const DevParams &p, int bs, hipStream_t st) {
-  hipLaunchKernelGGL(k_setup_nodes, dim3(nblk(p.nn, bs)), dim3(bs), 0, st, p, m.inc_off, m.inc, m.vol, m.mask, m.mass, m.gforce);
+  hipLaunchKernelGGL(k_setup_nodes, dim3(nblk(p.nn, bs)), dim3(bs), 0, st, p, m.inc_off, m.inc, m.vol, m.mask, m.mass,
+                     m.mnode, m.gforce);
 }
 template <typename R>
 void Launch<R>::tet(const DevMesh<R> &m, const TileDev &td, const DevParams &p, int bs, int force_hd, int write_e,
@@ -1024,12 +1026,12 @@ template <typename R>
 void Launch<R>::node_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st, hipEvent_t e0,
                           hipEvent_t e1) {
   launch_timed(k_node<R, false>, dim3(td.nblk), dim3(256), 0, st, e0, e1, p, m.ctrl, m.ring, td, m.fown, m.hpart,
-               m.fownh, m.hparth, m.hd, m.mass, m.mask, m.ext, m.xt);
+               m.fownh, m.hparth, m.hd, m.mnode, m.ext, m.xt);
 }
 template <typename R>
 void Launch<R>::grad_pass(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st) {
   hipLaunchKernelGGL((k_node<R, true>), dim3(td.nblk), dim3(256), 0, st, p, m.ctrl, m.ring, td, m.fown, m.hpart,
-                     m.fownh, m.hparth, m.hd, m.mass, m.mask, m.ext, m.xt);
+                     m.fownh, m.hparth, m.hd, m.mnode, m.ext, m.xt);
 }
 template <typename R>
 void Launch<R>::control(const DevMesh<R> &m, const TileDev &td, const DevParams &p, hipStream_t st, hipEvent_t e0,
