@@ -203,12 +203,16 @@ int pbs_tile_forces(pbs_solver *s, const float *pos, double *node_forces, double
 
 /* ---- measurement ---- */
 /* Average device time per launch (ms) of each kernel of the last pbs_step calls, measured with
- * HIP events on the handle's stream: names[i] / avg_ms[i] / launches[i] for i < *count. */
+ * HIP events on the handle's stream: names[i] / avg_ms[i] / launches[i] for i < *count.  The
+ * descent-pass kernels and the CG solves' SpMV and vector update are timed by events attached to
+ * their own dispatch (the interval a kernel trace reports); step start, warm start, end, the
+ * backward-Euler assembly and the PD local step by events recorded around them. */
 int pbs_kernel_times(pbs_solver *s, int max, const char **names, double *avg_ms, long *launches, int *count);
 int pbs_reset_kernel_times(pbs_solver *s);
 /* period 0: no timing (default); 1: every descent-pass kernel is timed; n > 1: the kernels of
- * every n-th graph replay (every n-th pass without graphs) are timed.  Launches that run after
- * the loop has stopped within a replay are not counted. */
+ * every n-th graph replay (every n-th pass without graphs) are timed, the timed replay's passes
+ * being launched one by one instead of from the graph.  Launches that run after the loop has
+ * stopped within a replay are not counted.  Any period > 0 times every CG replay. */
 int pbs_set_profiling(pbs_solver *s, int period);
 
 /* ---- backward Euler (the reference's GPU implicit path, fvm.cpp:179-183) ---- */
