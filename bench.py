@@ -184,6 +184,7 @@ def train_pooled_network(P, params, bar_dims, epochs=30, seed=7):
     x0 = P.mesh_twist(rest, *bar_dims, 0.05, 0.0, 0.002, seed)
     s = collect_trajectory(P, rest, tets, mask, x0, params, 30, seed, 4, 12_000_000)
     counts = {"bar64x64x256": s.learn_samples()}
+    progress("warm-start samples of the %dx%dx%d bar collected" % tuple(bar_dims))
     s.learn_fit_scales()
     s.learn_init_model(seed)
     loss_bar = s.learn_train(epochs, batch=8192, lr=2e-3, seed=seed)
@@ -265,6 +266,11 @@ def batch_report(P, grav, warmup=1, steps=3):
     return out
 
 
+def progress(msg):
+    """One line per finished part of the run, on stderr (stdout carries only the JSON line)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def warm_start_report(P, args, params, rest, tets, mask, x0, dims, plain_sps, plain_its, other=None):
     """Learned warm start vs plain with the pooled network (train_pooled_network):
     on the benchmark bar (the timed steps' own protocol: warm-up, then the timed steps) and on the
@@ -273,6 +279,7 @@ def warm_start_report(P, args, params, rest, tets, mask, x0, dims, plain_sps, pl
     out = {}
     model, model_bar, info = train_pooled_network(P, params, dims)
     out["training"] = info
+    progress("warm-start networks trained")
     lw = timed_steps(P, rest, tets, mask, x0, params, args.warmup, args.steps, model)
     out["bench_bar"] = {"plain_steps_per_s": plain_sps, "plain_iterations_per_step": plain_its,
                         "learned": lw, "speedup": lw["steps_per_s"] / plain_sps}
@@ -567,6 +574,7 @@ def main():
         out["autotune_us_per_pass"] = s.autotune_report() if params.tile_tets < 0 else None
         out["tile_tets"] = s.get_params().tile_tets
     s.close()
+    progress("timed steps done")
     if relg and f64 and not batch:
         # the same steps with the other arithmetic: mixed mode (FP32 per-tet arithmetic down to
         # |g|/|g0| = 2e-4) when the main run was FP64 throughout, and the reverse
@@ -575,6 +583,7 @@ def main():
         r = timed_steps(P, rest, tets, mask, x0, other_params, args.warmup, args.steps)
         r["mixed_switch"] = sw
         out["other_arithmetic"] = r
+        progress("other arithmetic done")
     if relg and not args.no_learned and not batch:
         other = (other_params, out["other_arithmetic"]) if f64 else None
         lw = warm_start_report(P, args, params, rest, tets, mask, x0, (nx, ny, nz), steps_per_s,
@@ -585,13 +594,17 @@ def main():
         out["learned_speedup_43k"] = lw["bar_12x12x50"]["speedup"]
         out["learned_speedup_bar_network"] = lw["bench_bar_bar_network"]["speedup"]
         out["learned_guess_used_fraction"] = lw["bench_bar"]["learned"]["guess_used_fraction"]
+        progress("warm start done")
     if not batch and args.workload == "bar64x64x256":
         out["fixed_64_iterations_f32"] = fixed_iteration_report(P, rest, tets, mask, grav)
+        progress("fixed-iteration runs done")
         if not args.no_batch:
             out["batch256"] = batch_report(P, grav)
+            progress("batch done")
     if not args.no_implicit and not batch:
         out["implicit_euler"] = implicit_report(P, rest, tets, mask, material, grav)
         out["projective_dynamics"] = pd_report(P, rest, tets, mask, grav)
+        progress("implicit and projective-dynamics steps done")
         if not args.no_cpu_baseline:
             out["implicit_euler"]["cpu_baseline"] = implicit_cpu_baseline(P, grav)
     if not args.no_cpu_baseline and batch:
